@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""Benchmark of the block-DDA solve on one MI355X.
+
+Workload (one "step" = one full solve): GRE particle bc = 0.8, ab = 0.7,
+beta = 0.2 (seed 0) on a 51 x 48 x 63 lattice padded to a 64^3 cuboid
+(128^3 doubled FFT grid), birefringent m_p = (1.55, 1.60, 1.50) + 0.001i,
+m_m = 1.33, L = 64 orientations, complex128 block-BiCGSTAB to relative
+residual 1e-5 with B resident on the device.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-mixed] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "block-dda_py_amd"))
+
+METRIC = "orientations solved/sec at rel. residual 1e-5 (mixed+warm vs plain FP64), MVP HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+PASSES = ["z_fwd", "y_fwd", "x_conv", "y_inv", "z_inv"]
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def pass_bytes(dims, m, N, L, Lb, s):
+    """Algorithmic HBM bytes of each FFT pass for one product (each pass reads
+    and writes its pruned array once; the x pass also reads the 6-component
+    half spectrum once)."""
+    nx, ny, nz = dims
+    mx, my, mz = m
+    Lp = -(-L // Lb) * Lb
+    c = 3 * Lp
+    t1 = nx * ny * mz * c
+    t2 = nx * my * mz * c
+    half = (mx // 2 + 1) * (my // 2 + 1) * (mz // 2 + 1) * 6
+    occ = 3 * N * L
+    return {
+        "z_fwd": s * (occ + t1),
+        "y_fwd": s * (t1 + t2),
+        "x_conv": s * (2 * t2 + half),
+        "y_inv": s * (t2 + t1),
+        "z_inv": s * (t1 + 2 * occ) + 2 * 16 * 3 * N,
+    }
+
+
+def choose_lb(m, L):
+    cap = max(1, min(4096 // (3 * max(m)), 16))
+    best, best_pad = 1, L
+    for lb in range(cap, 0, -1):
+        pad = -(-L // lb) * lb
+        if pad < best_pad or (pad == best_pad and lb > best):
+            best, best_pad = lb, pad
+    return best
+
+
+def timed_solves(dd, precision, steps, warmup, torch):
+    dd.precision = precision
+    for _ in range(warmup):
+        dd.solve_matrix_equation()
+    torch.cuda.synchronize()
+    dd._plan.profile_read()  # discard warm-up events
+    dd._plan.profile_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dd.solve_matrix_equation()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    prof = dd._plan.profile_read()
+    dd._plan.profile_enable(False)
+    return dt, prof
+
+
+def cpu_baseline(geometry, wl, gpu_iters, sample_L=16):
+    """The float64 NumPy/SciPy oracle (the reference algorithm) on the host:
+    one block-BiCGSTAB iteration at sample_L columns on the particle's own
+    (unpadded) lattice, scaled to L columns and the GPU iteration count."""
+    import numpy as np
+    cores = min(16, len(os.sched_getaffinity(0)))
+    os.environ["DDA_FFT_WORKERS"] = str(cores)
+    from oracle import bl_krylov as okr
+    from oracle import scatterer as osc
+    from bdda_amd.workloads import euler_angles, wavelength
+    name, n, lf, grid, is_in = geometry
+    m_p = np.array(wl["m_p_xyz"], dtype=np.complex128)
+    tg = osc.Target(n, lf, grid, is_in, m_p)
+    inc = osc.Incident(wavelength(wl, lf, tg.N), wl["m_m"], euler_angles(wl["L"], wl["orient_seed"])[:sample_L])
+    eps_r, alpha, diag = osc.polarizability(tg, inc)
+    from oracle.mvp_fft import fft_init
+    At = fft_init(tg.lattice_n, 3, tg.lf, inc.k)
+    E_inc, B = osc.rhs(tg, inc)
+    t0 = time.perf_counter()
+    okr.bl_bicgstab(tg.lattice_n, 3, tg.address, At, diag, B, 0.0, 1)
+    t_iter = time.perf_counter() - t0
+    per_iter_full = t_iter * wl["L"] / sample_L
+    total = per_iter_full * gpu_iters
+    return {"value": wl["L"] / total, "unit": "orientations/s", "cores": cores, "kind": "port",
+            "sample": f"1 block-BiCGSTAB iteration (float64 NumPy/SciPy oracle, {cores} threads) at L={sample_L} "
+                      f"on the particle's own {tuple(int(v) for v in n)} lattice ({t_iter:.2f} s), scaled x"
+                      f"{wl['L'] // sample_L} in L and x{gpu_iters} iterations (the GPU float64 count)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU layout is a design note, "
+              "not a measured path", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from bdda_amd import workloads
+    wl = workloads.FULL_64
+    log("building geometry")
+    geometry = workloads.build_geometry(wl)
+    log("setting interaction matrix")
+    dd = workloads.make_dipoles(wl, geometry=geometry, precision="fp64")
+    plan = dd._plan
+    N, L = dd.num_element_occupy, wl["L"]
+    dims, m = plan.n, plan.m
+    log(f"lattice {tuple(int(v) for v in dd.lattice_n)} cuboid {dims} fft {m} N={N} L={L}")
+
+    log(f"fp64 solves: warmup {a.warmup}, steps {a.steps}")
+    dt, prof = timed_solves(dd, "fp64", a.steps, a.warmup, torch)
+    iters = dd.iter_fin + 1
+    tr = dd.true_residual
+    log(f"fp64: {dt / a.steps * 1e3:.1f} ms/solve, {iters} iterations, true residual {tr:.2e}")
+
+    Lb = choose_lb(m, L)
+    pb = pass_bytes(dims, m, N, L, Lb, 16)
+    passes = {}
+    for nm in PASSES:
+        ms, cnt = prof.get(("fp64", nm), (0.0, 0))
+        if cnt:
+            avg = ms / cnt
+            passes[nm] = {"avg_ms": avg, "launches": cnt, "bytes": pb[nm], "GBs": pb[nm] / avg / 1e6}
+    dom = max(passes, key=lambda k: passes[k]["avg_ms"] * passes[k]["launches"])
+    d = passes[dom]
+    n_cells = int(np.prod(dims))
+    M = int(np.prod(m))
+    mvp_alg = 16 * (26 * n_cells * 3 * L + 6 * M)
+    mvp_ms = sum(p["avg_ms"] for p in passes.values())
+    roofline = {"bound": "hbm", "kernel": f"k_line_pass<double2, {dom}>", "achieved": round(d["GBs"], 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(d["GBs"] / HBM_PEAK_GBS, 4),
+                "traffic": None, "bytes_per_launch": d["bytes"], "avg_launch_ms": round(d["avg_ms"], 4)}
+
+    result = {
+        "metric": METRIC, "value": round(L * a.steps / dt, 3), "unit": "orientations/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(dt / a.steps * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": wl["name"], "lattice": [int(v) for v in dd.lattice_n], "cuboid": list(dims),
+                   "fft_grid": list(m), "dipoles": int(N), "L": L, "solver": wl["solver"], "tol": wl["tol"],
+                   "precision": "fp64"},
+        "iterations": iters, "true_residual": tr,
+        "roofline": roofline,
+        "mvp": {"ms": round(mvp_ms, 4), "alg_bytes": mvp_alg, "alg_GBs": round(mvp_alg / mvp_ms / 1e6, 1),
+                "passes": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                           for k, v in passes.items()}},
+    }
+
+    if not a.no_mixed:
+        S_fw64 = np.stack(dd.compute_PCAS_observable_S_fw())
+        S_bk64 = dd.compute_OCBS_observable_S_bk()
+        log("mixed solves")
+        dtm, profm = timed_solves(dd, "mixed", a.steps, a.warmup, torch)
+        S_fwm = np.stack(dd.compute_PCAS_observable_S_fw())
+        S_bkm = dd.compute_OCBS_observable_S_bk()
+        pbm = pass_bytes(dims, m, N, L, choose_lb(m, L), 8)
+        mp = {}
+        for nm in PASSES:
+            ms, cnt = profm.get(("fp32", nm), (0.0, 0))
+            if cnt:
+                mp[nm] = {"avg_ms": round(ms / cnt, 4), "launches": cnt, "GBs": round(pbm[nm] / (ms / cnt) / 1e6, 1)}
+        result["mixed"] = {
+            "value": round(L * a.steps / dtm, 3), "ms_per_step": round(dtm / a.steps * 1e3, 2),
+            "iterations": dd.iter_fin + 1, "true_residual": dd.true_residual,
+            "speedup_vs_fp64": round(dt / dtm, 3),
+            "S_fw_max_rel_err_vs_fp64": float(np.max(np.abs(S_fwm - S_fw64) / np.abs(S_fw64))),
+            "S_bk_max_rel_err_vs_fp64": float(np.max(np.abs(S_bkm - S_bk64) / np.abs(S_bk64))),
+            "passes_c64": mp,
+        }
+        log(f"mixed: {dtm / a.steps * 1e3:.1f} ms/solve")
+
+    if not a.no_cpu_baseline:
+        log("cpu baseline")
+        result["cpu_baseline"] = cpu_baseline(geometry, wl, iters)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,71 @@
+"""Synthetic benchmark workloads (GRE particles, orientations, materials).
+
+Each workload is a plain dict of shape, material and solver settings; the
+geometry is built by the drop-in GRE model (host surface sampling, GPU
+voxeliser) and the orientations follow the reference's draw order
+(``run_dda.py:46-54``: alpha, beta, gamma uniform from one generator).
+"""
+import numpy as np
+
+# 64^3 cuboid: GRE bc = 0.8, ab = 0.7, beta = 0.2 (seed 0).  r_v gives a
+# 51 x 48 x 63 lattice (21,113 dipoles) padded with vacuum to 64^3, i.e. a
+# 128^3 doubled FFT grid; birefringent m_p; L = 64; BiCGSTAB to 1e-5.
+FULL_64 = dict(
+    name="gre_bc0.8_ab0.7_beta0.2_seed0_64cube_L64",
+    r_v_base=1.0107600709889086, bc_ratio=0.8, ab_ratio=0.7, beta=0.2, shape_seed=0,
+    m_p_xyz=(1.55 + 0.001j, 1.60 + 0.001j, 1.50 + 0.001j), m_m=1.33, dpl=15.0,
+    L=64, orient_seed=1, pad_to=(64, 64, 64), solver="bicgstab", tol=1e-5, itermax=500,
+)
+
+# 24^3 isotropic sphere-like GRE (beta = 0.1, seed 0), m_p = 1.5 + 0.01i,
+# m_m = 1, size parameter ~3, L = 8, COCG-rq to 1e-5.
+SMALL_24 = dict(
+    name="gre_bc1_ab1_beta0.1_seed0_24cube_L8",
+    r_v_base=0.35858192704617975, bc_ratio=1.0, ab_ratio=1.0, beta=0.1, shape_seed=0,
+    m_p_xyz=(1.5 + 0.01j,) * 3, m_m=1.0, size_parameter=3.0,
+    L=8, orient_seed=1, pad_to=None, solver="cocg_rq", tol=1e-5, itermax=500,
+)
+
+
+def euler_angles(L, seed):
+    rng = np.random.default_rng(seed)
+    return np.column_stack([rng.uniform(0, 2 * np.pi, L), rng.uniform(0, np.pi, L), rng.uniform(0, 2 * np.pi, L)])
+
+
+def build_geometry(wl):
+    """(name, lattice_n, lf, grid_points, is_in) via the GPU voxeliser."""
+    from shape_model.gaussian_ellipsoid import gaussian_ellipsoid_shape_model as G
+    g = G(wl["r_v_base"], wl["bc_ratio"], wl["ab_ratio"], wl["beta"])
+    pts, _ = g.compute_r_points_on_GRE(np.random.default_rng(wl["shape_seed"]))
+    _, n, grid = g.create_cuboid_lattice_that_encloses_GRE_shape(pts)
+    dist = g.find_nearest_distance_from_the_GRE_surf(grid, pts)
+    is_in = g.extract_lattice_address_in_GRE_volume(g.lattice_lf, g.distance_factor, n, dist)
+    return g.name, n, g.lattice_lf, grid, is_in
+
+
+def wavelength(wl, lf, n_occ):
+    """Vacuum wavelength from the workload's dpl or size parameter."""
+    m_p = np.asarray(wl["m_p_xyz"])
+    if "dpl" in wl:
+        return wl["dpl"] * lf * np.abs(np.max(m_p))
+    r_ve = np.cbrt(3 * lf ** 3 * n_occ / (4 * np.pi))
+    return 2 * np.pi * wl["m_m"] * r_ve / wl["size_parameter"]
+
+
+def make_dipoles(wl, geometry=None, precision="fp64", solver=None, verbose=False):
+    """DiscreteDipoles for a workload, interaction matrix and B set on the device."""
+    from bl_dda.scatterer import DiscreteDipoles, IncidentField, Target
+    name, n, lf, grid, is_in = geometry if geometry is not None else build_geometry(wl)
+    m_p = np.array(wl["m_p_xyz"], dtype=np.complex128)
+    tg = Target(name, n, lf, grid, is_in, m_p)
+    wl0 = wavelength(wl, lf, tg.num_element_occupy)
+    inc = IncidentField(wl0, wl["m_m"], euler_angles(wl["L"], wl["orient_seed"]))
+    dd = DiscreteDipoles(tg, inc)
+    dd.verbose = verbose
+    dd.solver = solver or wl["solver"]
+    dd.precision = precision
+    dd.tol = wl["tol"]
+    dd.itermax = wl["itermax"]
+    dd.pad_to = wl["pad_to"]
+    dd.set_interaction_matrix()
+    return dd

@@ -1,0 +1,131 @@
+// Internal declarations shared by the kernel and host translation units of
+// libbdda (MI355X / gfx950).  Not part of the public C ABI (include/bdda.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace bdda {
+
+// ----------------------------------------------------------------- errors
+void set_error(const std::string& msg);
+#define BDDA_HIP(call)                                                        \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      ::bdda::set_error(std::string(#call) + ": " + hipGetErrorString(e_));   \
+      return 2;                                                               \
+    }                                                                         \
+  } while (0)
+
+// ------------------------------------------------------------ FFT plans
+constexpr int kMaxRadix = 12;  // passes per axis
+
+// One axis of the doubled (convolution) grid.
+struct AxisPlan {
+  int n = 0;        // cuboid cells on this axis (N)
+  int m = 0;        // FFT length (>= 2N-1, 5-smooth)
+  int h = 0;        // m/2 + 1: stored half-spectrum of the Green tensor
+  int nrad = 0;
+  int rad[kMaxRadix] = {0};
+};
+
+// Radix sequence for a 5-smooth length (radices 8, 4, 2, 3, 5).
+bool factor_length(int m, AxisPlan& ax);
+int smallest_5smooth_at_least(int v);
+
+// Launch-time description of one batched line-FFT pass.  Every launch moves
+// `npts` points (lines) x Q columns of length-m transforms through LDS.
+enum PassMode : int {
+  PASS_PLAIN = 0,       // global -> FFT -> global (strided lines)
+  PASS_SCATTER_Z = 1,   // occupied block (3N,L) -> z FFT -> T1
+  PASS_CONV_X = 2,      // x FFT -> 3x3 Green multiply -> inverse x FFT
+  PASS_GATHER_Z = 3,    // T1 -> inverse z FFT -> diag + Jacobi epilogue -> (3N,L)
+  PASS_CUBOID_Z = 4,    // cuboid (3n,L) -> z FFT -> T1
+  PASS_CROP_Z = 5,      // T1 -> inverse z FFT -> cuboid (3n,L) (fft_mvp)
+};
+
+struct LinePass {
+  int mode;
+  int inverse;
+  int m, nin, nout;
+  int nrad;
+  int rad[kMaxRadix];
+  const void* tw;       // twiddle table of length m (forward sign), V-typed
+  int Q;                // columns per point (3*Lb, or 6 for the Green tensor)
+  int npts;             // number of points (lines)
+  int ptb;              // point index = a*ptb + b
+  long long sa, sb;     // point offsets (complex units)  -- source
+  long long se;         // element stride (complex units)  -- source
+  long long da, db, de; // same for destination
+  long long lo_src, lo_dst;  // column-tile strides (complex units)
+  int P;                // points per workgroup
+  int order_pe;         // flat tile order: 1 -> (q, p, e), 0 -> (q, e, p)
+  const void* src;
+  void* dst;
+  // scatter / gather (solver layout)
+  const int* inv;       // cell -> occupied index or -1
+  int nyz_y, nz;        // cuboid Ny, Nz (z-line points are (ix, iy))
+  int L, Lb;            // total columns, column tile
+  int nlo;              // number of column tiles (grid.y)
+  const void* S;        // occupied block (3N, L) (scatter source / diag term)
+  void* AS;             // occupied output (3N, L)
+  const void* diag;     // (3N) complex
+  const void* jpre;     // (3N) complex
+  double scale;         // 1/M for the inverse transform
+  // Green multiply
+  const void* green;    // (hx, hy, hz, 6) half spectrum
+  int hx, hy, hz;
+  int mx, my, mz;
+};
+
+template <typename V>
+void launch_line_pass(const LinePass& lp, hipStream_t s);
+template <typename V>
+void launch_green_build(V* out, const AxisPlan* ax, double lf, double k, hipStream_t s);
+template <typename V>
+void launch_green_halve(const double2* full, V* half, const AxisPlan* ax, hipStream_t s);
+void launch_green_sym9(const double2* g6, double2* g9, long long M, hipStream_t s);
+void launch_green_expand(const double2* half, double2* full, const AxisPlan* ax, const int* n2,
+                         hipStream_t s);
+
+// --------------------------------------------------------- block kernels
+// Gram: G[j][l] = sum_i op(A[i][j]) * B[i][l]   (op = conj or identity), FP64 out.
+template <typename V>
+void launch_gram(const V* A, const V* B, long long rows, int L, bool conjA, double2* partial,
+                 int nparts, double2* out, hipStream_t s);
+int gram_parts();
+// Y = a*X + sum_j (W1 + w2c*W2)[i][j] C[j][l] + e*E ;  X, W2, E nullable.
+template <typename V>
+void launch_update(V* Y, const V* X, double2 a, const V* W1, const V* W2, double2 w2c,
+                   const double2* C, const V* E, double2 e, long long rows, int L, hipStream_t s);
+// Y = X + c*Z, and partial |Y|^2 sums (nullable).
+template <typename V>
+void launch_axpy_norm(V* Y, const V* X, double2 c, const V* Z, long long n, double* partial,
+                      int nparts, hipStream_t s);
+// partial sums of tr(A^H B) and |B|^2 (each nullable) -> double2/double partials
+template <typename V>
+void launch_dot(const V* A, const V* B, long long n, double2* part_dot, double* part_nrm,
+                int nparts, hipStream_t s);
+int dot_parts();
+template <typename V>
+void launch_scale_rows(V* Y, const V* X, const double2* d, long long rows, int L, hipStream_t s);
+template <typename Vd, typename Vs>
+void launch_convert(Vd* dst, const Vs* src, long long n, double scale, hipStream_t s);
+void launch_add_c64_to_c128(double2* X, const float2* D, long long n, double scale, hipStream_t s);
+void launch_fill_inv(int* inv, long long ncell, const long long* addr, long long N, hipStream_t s);
+
+// physics
+void launch_rhs(double2* B, const double* pos, const double* u, const double2* e0, double k,
+                long long N, int L, hipStream_t s);
+void launch_observables(const double2* X, const double* pos, const double* geo, const double2* epsfac,
+                        double k, long long N, int L, double2* partial, int nparts, hipStream_t s);
+int obs_parts();
+void launch_voxel_dist(const double* grid, long long n, const double* surf, long long P, double* dist,
+                       hipStream_t s);
+void launch_voxel_inside(const double* dist, double thresh, int nx, int ny, int nz, unsigned char* inside,
+                         hipStream_t s);
+
+}  // namespace bdda

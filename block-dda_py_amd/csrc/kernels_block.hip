@@ -1,0 +1,544 @@
+// Tall-skinny block kernels of the block-Krylov solvers, the right-hand side
+// and observable reductions, and the GRE voxeliser, for gfx950.
+//
+// Krylov blocks are (rows, L) row-major with rows = 3N (row = 3m + c) and the
+// L orientation columns innermost, so a wavefront reads 64 consecutive
+// columns of one row.  Small L x L coefficient matrices are complex float64.
+#include "internal.h"
+
+namespace bdda {
+
+template <typename V> struct RealOf;
+template <> struct RealOf<double2> { using T = double; };
+template <> struct RealOf<float2> { using T = float; };
+
+__device__ __forceinline__ double2 d2(float2 v) { return make_double2(v.x, v.y); }
+__device__ __forceinline__ double2 d2(double2 v) { return v; }
+template <typename V> __device__ __forceinline__ V from_d2(double2 v);
+template <> __device__ __forceinline__ double2 from_d2<double2>(double2 v) { return v; }
+template <> __device__ __forceinline__ float2 from_d2<float2>(double2 v) {
+  return make_float2((float)v.x, (float)v.y);
+}
+__device__ __forceinline__ double2 zmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 zadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// --------------------------------------------------------------------- Gram
+// partial[wg][tile](64x64) = sum_{rows of wg} op(A[i][jt*64+a]) * B[i][lt*64+b]
+constexpr int kGramParts = 512;
+int gram_parts() { return kGramParts; }
+
+template <typename V, bool CONJ>
+__global__ void __launch_bounds__(256) k_gram(const V* __restrict__ A, const V* __restrict__ B, long long rows,
+                                              int L, double2* __restrict__ partial) {
+  __shared__ double2 As[16][64];
+  __shared__ double2 Bs[16][64];
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  const int ntile = (L + 63) / 64;
+  const int jt = blockIdx.y / ntile, lt = blockIdx.y % ntile;
+  const long long chunk = (rows + gridDim.x - 1) / gridDim.x;
+  const long long r0 = blockIdx.x * chunk;
+  const long long r1 = min(rows, r0 + chunk);
+  double2 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = make_double2(0.0, 0.0);
+  for (long long rs = r0; rs < r1; rs += 16) {
+    for (int idx = threadIdx.x; idx < 16 * 64; idx += 256) {
+      const int rr = idx / 64, cc = idx % 64;
+      const long long row = rs + rr;
+      const int ja = jt * 64 + cc, lb = lt * 64 + cc;
+      double2 va = make_double2(0.0, 0.0), vb = make_double2(0.0, 0.0);
+      if (row < r1) {
+        if (ja < L) va = d2(A[row * L + ja]);
+        if (lb < L) vb = d2(B[row * L + lb]);
+      }
+      if (CONJ) va.y = -va.y;
+      As[rr][cc] = va;
+      Bs[rr][cc] = vb;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int rr = 0; rr < 16; ++rr) {
+      double2 a[4], b[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        a[t] = As[rr][ty * 4 + t];
+        b[t] = Bs[rr][tx * 4 + t];
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          acc[i][j].x += a[i].x * b[j].x - a[i].y * b[j].y;
+          acc[i][j].y += a[i].x * b[j].y + a[i].y * b[j].x;
+        }
+    }
+    __syncthreads();
+  }
+  double2* out = partial + ((long long)blockIdx.x * gridDim.y + blockIdx.y) * 4096;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) out[(ty * 4 + i) * 64 + tx * 4 + j] = acc[i][j];
+}
+
+__global__ void k_gram_reduce(const double2* __restrict__ partial, int nparts, int ntile2, int L,
+                              double2* __restrict__ out) {
+  const int ntile = (L + 63) / 64;
+  const long long total = (long long)ntile2 * 4096;
+  for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
+       id += (long long)gridDim.x * blockDim.x) {
+    const int tile = id / 4096, w = id % 4096;
+    const int jt = tile / ntile, lt = tile % ntile;
+    const int j = jt * 64 + w / 64, l = lt * 64 + w % 64;
+    if (j >= L || l >= L) continue;
+    double2 s = make_double2(0.0, 0.0);
+    for (int p = 0; p < nparts; ++p) s = zadd(s, partial[((long long)p * ntile2 + tile) * 4096 + w]);
+    out[j * L + l] = s;
+  }
+}
+
+template <typename V>
+void launch_gram(const V* A, const V* B, long long rows, int L, bool conjA, double2* partial, int nparts,
+                 double2* out, hipStream_t s) {
+  const int ntile = (L + 63) / 64;
+  dim3 grid(nparts, ntile * ntile);
+  if (conjA)
+    hipLaunchKernelGGL((k_gram<V, true>), grid, dim3(256), 0, s, A, B, rows, L, partial);
+  else
+    hipLaunchKernelGGL((k_gram<V, false>), grid, dim3(256), 0, s, A, B, rows, L, partial);
+  hipLaunchKernelGGL(k_gram_reduce, dim3(64), dim3(256), 0, s, partial, nparts, ntile * ntile, L, out);
+}
+template void launch_gram<double2>(const double2*, const double2*, long long, int, bool, double2*, int, double2*,
+                                   hipStream_t);
+template void launch_gram<float2>(const float2*, const float2*, long long, int, bool, double2*, int, double2*,
+                                  hipStream_t);
+
+// ------------------------------------------------------------ block update
+// Y = a X + (W1 + w2c W2) C + e E  over 16-row slabs, output columns tiled by 64.
+template <typename V>
+__global__ void __launch_bounds__(256) k_update(V* Y, const V* X, double2 a, const V* W1, const V* W2,
+                                                double2 w2c, const double2* __restrict__ C, const V* E,
+                                                double2 e, long long rows, int L) {
+  using T = typename RealOf<V>::T;
+  extern __shared__ __align__(16) unsigned char sm[];
+  V* Cs = reinterpret_cast<V*>(sm);      // [L][64]
+  V* Ws = Cs + (size_t)L * 64;           // [16][L]
+  const int lt = blockIdx.y;
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  for (int idx = threadIdx.x; idx < L * 64; idx += 256) {
+    const int j = idx / 64, c = idx % 64;
+    const int l = lt * 64 + c;
+    Cs[idx] = from_d2<V>(l < L ? C[j * L + l] : make_double2(0.0, 0.0));
+  }
+  const V wc = from_d2<V>(w2c);
+  for (long long r0 = blockIdx.x * 16LL; r0 < rows; r0 += 16LL * gridDim.x) {
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < 16 * L; idx += 256) {
+      const int rr = idx / L, j = idx % L;
+      const long long row = r0 + rr;
+      V w;
+      w.x = 0;
+      w.y = 0;
+      if (row < rows) {
+        w = W1[row * L + j];
+        if (W2) {
+          const V w2 = W2[row * L + j];
+          w.x += wc.x * w2.x - wc.y * w2.y;
+          w.y += wc.x * w2.y + wc.y * w2.x;
+        }
+      }
+      Ws[idx] = w;
+    }
+    __syncthreads();
+    const long long row = r0 + ty;
+    V acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      acc[t].x = 0;
+      acc[t].y = 0;
+    }
+    const V* wr = Ws + ty * L;
+    for (int j = 0; j < L; ++j) {
+      const V w = wr[j];
+      const V* cr = Cs + j * 64 + tx * 4;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const V c = cr[t];
+        acc[t].x += w.x * c.x - w.y * c.y;
+        acc[t].y += w.x * c.y + w.y * c.x;
+      }
+    }
+    if (row < rows) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int l = lt * 64 + tx * 4 + t;
+        if (l >= L) continue;
+        const long long o = row * L + l;
+        V y = acc[t];
+        if (X) {
+          const V x = X[o];
+          y.x += (T)a.x * x.x - (T)a.y * x.y;
+          y.y += (T)a.x * x.y + (T)a.y * x.x;
+        }
+        if (E) {
+          const V ev = E[o];
+          y.x += (T)e.x * ev.x - (T)e.y * ev.y;
+          y.y += (T)e.x * ev.y + (T)e.y * ev.x;
+        }
+        Y[o] = y;
+      }
+    }
+  }
+}
+
+template <typename V>
+void launch_update(V* Y, const V* X, double2 a, const V* W1, const V* W2, double2 w2c, const double2* C,
+                   const V* E, double2 e, long long rows, int L, hipStream_t s) {
+  const size_t lds = ((size_t)L * 64 + 16 * (size_t)L) * sizeof(V);
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_update<V>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr = true;
+  }
+  long long slabs = (rows + 15) / 16;
+  int g = (int)min(slabs, 4096LL);
+  dim3 grid(g, (L + 63) / 64);
+  hipLaunchKernelGGL((k_update<V>), grid, dim3(256), lds, s, Y, X, a, W1, W2, w2c, C, E, e, rows, L);
+}
+template void launch_update<double2>(double2*, const double2*, double2, const double2*, const double2*, double2,
+                                     const double2*, const double2*, double2, long long, int, hipStream_t);
+template void launch_update<float2>(float2*, const float2*, double2, const float2*, const float2*, double2,
+                                    const double2*, const float2*, double2, long long, int, hipStream_t);
+
+// ------------------------------------------------------------ reductions
+constexpr int kDotParts = 1024;
+int dot_parts() { return kDotParts; }
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < (int)(blockDim.x / 64); ++i) t += sh[i];
+  }
+  __syncthreads();
+  return t;
+}
+
+template <typename V>
+__global__ void __launch_bounds__(256) k_axpy_norm(V* Y, const V* X, double2 c, const V* Z, long long n,
+                                                   double* partial) {
+  using T = typename RealOf<V>::T;
+  __shared__ double sh[4];
+  double s = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const V x = X[i], z = Z[i];
+    V y;
+    y.x = x.x + (T)c.x * z.x - (T)c.y * z.y;
+    y.y = x.y + (T)c.x * z.y + (T)c.y * z.x;
+    Y[i] = y;
+    s += (double)y.x * y.x + (double)y.y * y.y;
+  }
+  const double t = block_sum(s, sh);
+  if (partial && threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+template <typename V>
+void launch_axpy_norm(V* Y, const V* X, double2 c, const V* Z, long long n, double* partial, int nparts,
+                      hipStream_t s) {
+  hipLaunchKernelGGL((k_axpy_norm<V>), dim3(nparts), dim3(256), 0, s, Y, X, c, Z, n, partial);
+}
+template void launch_axpy_norm<double2>(double2*, const double2*, double2, const double2*, long long, double*, int,
+                                        hipStream_t);
+template void launch_axpy_norm<float2>(float2*, const float2*, double2, const float2*, long long, double*, int,
+                                       hipStream_t);
+
+template <typename V>
+__global__ void __launch_bounds__(256) k_dot(const V* A, const V* B, long long n, double2* pdot, double* pnrm) {
+  __shared__ double sh[4];
+  double sr = 0.0, si = 0.0, sn = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double2 b = d2(B[i]);
+    if (pdot) {
+      const double2 a = d2(A[i]);
+      sr += a.x * b.x + a.y * b.y;
+      si += a.x * b.y - a.y * b.x;
+    }
+    sn += b.x * b.x + b.y * b.y;
+  }
+  const double tr = block_sum(sr, sh);
+  const double ti = block_sum(si, sh);
+  const double tn = block_sum(sn, sh);
+  if (threadIdx.x == 0) {
+    if (pdot) pdot[blockIdx.x] = make_double2(tr, ti);
+    if (pnrm) pnrm[blockIdx.x] = tn;
+  }
+}
+
+template <typename V>
+void launch_dot(const V* A, const V* B, long long n, double2* pdot, double* pnrm, int nparts, hipStream_t s) {
+  hipLaunchKernelGGL((k_dot<V>), dim3(nparts), dim3(256), 0, s, A, B, n, pdot, pnrm);
+}
+template void launch_dot<double2>(const double2*, const double2*, long long, double2*, double*, int, hipStream_t);
+template void launch_dot<float2>(const float2*, const float2*, long long, double2*, double*, int, hipStream_t);
+
+// ------------------------------------------------------------ elementwise
+template <typename V>
+__global__ void k_scale_rows(V* Y, const V* X, const double2* d, long long rows, int L) {
+  const long long n = rows * L;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    Y[i] = from_d2<V>(zmul(d[i / L], d2(X[i])));
+  }
+}
+template <typename V>
+void launch_scale_rows(V* Y, const V* X, const double2* d, long long rows, int L, hipStream_t s) {
+  hipLaunchKernelGGL((k_scale_rows<V>), dim3(2048), dim3(256), 0, s, Y, X, d, rows, L);
+}
+template void launch_scale_rows<double2>(double2*, const double2*, const double2*, long long, int, hipStream_t);
+template void launch_scale_rows<float2>(float2*, const float2*, const double2*, long long, int, hipStream_t);
+
+template <typename Vd, typename Vs>
+__global__ void k_convert(Vd* dst, const Vs* src, long long n, double scale) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double2 v = d2(src[i]);
+    dst[i] = from_d2<Vd>(make_double2(v.x * scale, v.y * scale));
+  }
+}
+template <typename Vd, typename Vs>
+void launch_convert(Vd* dst, const Vs* src, long long n, double scale, hipStream_t s) {
+  hipLaunchKernelGGL((k_convert<Vd, Vs>), dim3(2048), dim3(256), 0, s, dst, src, n, scale);
+}
+template void launch_convert<float2, double2>(float2*, const double2*, long long, double, hipStream_t);
+template void launch_convert<double2, float2>(double2*, const float2*, long long, double, hipStream_t);
+template void launch_convert<double2, double2>(double2*, const double2*, long long, double, hipStream_t);
+template void launch_convert<float2, float2>(float2*, const float2*, long long, double, hipStream_t);
+
+__global__ void k_add_c64(double2* X, const float2* D, long long n, double scale) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float2 d = D[i];
+    X[i].x += scale * (double)d.x;
+    X[i].y += scale * (double)d.y;
+  }
+}
+void launch_add_c64_to_c128(double2* X, const float2* D, long long n, double scale, hipStream_t s) {
+  hipLaunchKernelGGL(k_add_c64, dim3(2048), dim3(256), 0, s, X, D, n, scale);
+}
+
+__global__ void k_fill_int(int* p, long long n, int v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void k_scatter_inv(int* inv, const long long* addr, long long N) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < N;
+       i += (long long)gridDim.x * blockDim.x)
+    inv[addr[i]] = (int)i;
+}
+void launch_fill_inv(int* inv, long long ncell, const long long* addr, long long N, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_int, dim3(1024), dim3(256), 0, s, inv, ncell, -1);
+  hipLaunchKernelGGL(k_scatter_inv, dim3(1024), dim3(256), 0, s, inv, addr, N);
+}
+
+// -------------------------------------------------------------- physics
+// B[3m+c][l] = exp(i k u_l . r_m) e0[l][c]
+__global__ void k_rhs(double2* B, const double* pos, const double* u, const double2* e0, double k, long long N,
+                      int L) {
+  const long long n = N * L;
+  for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < n;
+       id += (long long)gridDim.x * blockDim.x) {
+    const long long m = id / L;
+    const int l = id % L;
+    const double ph = k * (u[3 * l] * pos[3 * m] + u[3 * l + 1] * pos[3 * m + 1] + u[3 * l + 2] * pos[3 * m + 2]);
+    double sn, cs;
+    sincos(ph, &sn, &cs);
+    const double2 e = make_double2(cs, sn);
+    for (int c = 0; c < 3; ++c) B[(3 * m + c) * L + l] = zmul(e, e0[3 * l + c]);
+  }
+}
+void launch_rhs(double2* B, const double* pos, const double* u, const double2* e0, double k, long long N, int L,
+                hipStream_t s) {
+  hipLaunchKernelGGL(k_rhs, dim3(2048), dim3(256), 0, s, B, pos, u, e0, k, N, L);
+}
+
+// Per orientation l and per workgroup: sum over its dipoles of
+//  [0] P.th_fw conj(e_fw)   [1] P.ph_fw (-i) conj(e_fw)
+//  [2] P.th_bk conj(e_bk)   [3] P.ph_bk (-i) conj(e_bk)
+//  [4] (sum_c Im(P_c conj(P_c f_c)), sum_c Im(P_c conj(e_fw e0_c)))
+// geo per l: u_fw(3) th_fw(3) ph_fw(3) u_bk(3) th_bk(3) ph_bk(3) e0 (3 complex = 6)
+constexpr int kObsParts = 256;
+int obs_parts() { return kObsParts; }
+
+__global__ void __launch_bounds__(256) k_observables(const double2* __restrict__ X, const double* __restrict__ pos,
+                                                     const double* __restrict__ geo,
+                                                     const double2* __restrict__ epsfac, double k, long long N,
+                                                     int L, double2* __restrict__ partial) {
+  __shared__ double2 red[4][64][5];
+  const int tl = threadIdx.x % 64, tm = threadIdx.x / 64;
+  for (int l0 = 0; l0 < L; l0 += 64) {
+    const int l = l0 + tl;
+    double2 acc[5];
+    for (int t = 0; t < 5; ++t) acc[t] = make_double2(0.0, 0.0);
+    if (l < L) {
+      const double* g = geo + 24 * l;
+      for (long long m = (long long)blockIdx.x * 4 + tm; m < N; m += 4LL * gridDim.x) {
+        const double rx = pos[3 * m], ry = pos[3 * m + 1], rz = pos[3 * m + 2];
+        double2 P[3];
+        for (int c = 0; c < 3; ++c) P[c] = X[(3 * m + c) * L + l];
+        double sn, cs;
+        sincos(k * (g[0] * rx + g[1] * ry + g[2] * rz), &sn, &cs);
+        const double2 efw = make_double2(cs, sn);   // e_fw (= e_inc phase)
+        const double2 cef = make_double2(cs, -sn);  // conj(e_fw)
+        sincos(k * (g[9] * rx + g[10] * ry + g[11] * rz), &sn, &cs);
+        const double2 ceb = make_double2(cs, -sn);  // conj(e_bk)
+        double2 pt = make_double2(0, 0), pp = make_double2(0, 0), bt = make_double2(0, 0), bp = make_double2(0, 0);
+        for (int c = 0; c < 3; ++c) {
+          pt.x += P[c].x * g[3 + c];
+          pt.y += P[c].y * g[3 + c];
+          pp.x += P[c].x * g[6 + c];
+          pp.y += P[c].y * g[6 + c];
+          bt.x += P[c].x * g[12 + c];
+          bt.y += P[c].y * g[12 + c];
+          bp.x += P[c].x * g[15 + c];
+          bp.y += P[c].y * g[15 + c];
+        }
+        acc[0] = zadd(acc[0], zmul(pt, cef));
+        acc[1] = zadd(acc[1], zmul(zmul(pp, make_double2(0.0, -1.0)), cef));
+        acc[2] = zadd(acc[2], zmul(bt, ceb));
+        acc[3] = zadd(acc[3], zmul(zmul(bp, make_double2(0.0, -1.0)), ceb));
+        for (int c = 0; c < 3; ++c) {
+          const double2 E = zmul(P[c], epsfac[c]);
+          acc[4].x += P[c].y * E.x - P[c].x * E.y;  // Im(P conj(E))
+          const double2 Ei = zmul(efw, make_double2(g[18 + 2 * c], g[19 + 2 * c]));
+          acc[4].y += P[c].y * Ei.x - P[c].x * Ei.y;  // Im(P conj(E_inc))
+        }
+      }
+    }
+    for (int t = 0; t < 5; ++t) red[tm][tl][t] = acc[t];
+    __syncthreads();
+    if (tm == 0 && l < L) {
+      for (int t = 0; t < 5; ++t) {
+        double2 s = red[0][tl][t];
+        for (int w = 1; w < 4; ++w) s = zadd(s, red[w][tl][t]);
+        partial[((long long)blockIdx.x * L + l) * 5 + t] = s;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+void launch_observables(const double2* X, const double* pos, const double* geo, const double2* epsfac, double k,
+                        long long N, int L, double2* partial, int nparts, hipStream_t s) {
+  hipLaunchKernelGGL(k_observables, dim3(nparts), dim3(256), 0, s, X, pos, geo, epsfac, k, N, L, partial);
+}
+
+// ---------------------------------------------------------------- voxeliser
+// dist[i] = sqrt(min_j ((dx*dx + dy*dy) + dz*dz)), evaluated without
+// contraction so that every squared distance is the KD-tree's bit for bit
+// (the minimum is order-free and sqrt is correctly rounded).
+#pragma clang fp contract(off)
+__global__ void __launch_bounds__(256) k_voxel_dist(const double* __restrict__ grid, long long n,
+                                                    const double* __restrict__ surf, long long P,
+                                                    double* __restrict__ dist) {
+  __shared__ double sp[3][1024];
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  double gx = 0, gy = 0, gz = 0;
+  if (i < n) {
+    gx = grid[3 * i];
+    gy = grid[3 * i + 1];
+    gz = grid[3 * i + 2];
+  }
+  double best = 1.0e300;
+  for (long long j0 = 0; j0 < P; j0 += 1024) {
+    const int cnt = (int)min(1024LL, P - j0);
+    __syncthreads();
+    for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+      sp[0][t] = surf[3 * (j0 + t)];
+      sp[1][t] = surf[3 * (j0 + t) + 1];
+      sp[2][t] = surf[3 * (j0 + t) + 2];
+    }
+    __syncthreads();
+    for (int t = 0; t < cnt; ++t) {
+      const double dx = gx - sp[0][t];
+      const double dy = gy - sp[1][t];
+      const double dz = gz - sp[2][t];
+      double r2 = dx * dx;
+      r2 = r2 + dy * dy;
+      r2 = r2 + dz * dz;
+      best = r2 < best ? r2 : best;
+    }
+  }
+  if (i < n) dist[i] = sqrt(best);
+}
+#pragma clang fp contract(on)
+
+void launch_voxel_dist(const double* grid, long long n, const double* surf, long long P, double* dist,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_voxel_dist, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, grid, n, surf, P, dist);
+}
+
+// inside along an axis <=> a near voxel lies strictly before and strictly
+// after it on that line (first_near < i < last_near); AND over the 3 axes.
+__global__ void k_voxel_inside(const double* __restrict__ dist, double thresh, int nx, int ny, int nz,
+                               unsigned char* __restrict__ inside) {
+  const long long lines_z = (long long)nx * ny, lines_y = (long long)nx * nz, lines_x = (long long)ny * nz;
+  const long long total = lines_z + lines_y + lines_x;
+  for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
+       id += (long long)gridDim.x * blockDim.x) {
+    long long base, stride;
+    int len;
+    if (id < lines_z) {
+      base = id * nz;
+      stride = 1;
+      len = nz;
+    } else if (id < lines_z + lines_y) {
+      const long long t = id - lines_z;
+      const long long ix = t / nz, iz = t % nz;
+      base = ix * (long long)ny * nz + iz;
+      stride = nz;
+      len = ny;
+    } else {
+      const long long t = id - lines_z - lines_y;
+      base = t;
+      stride = (long long)ny * nz;
+      len = nx;
+    }
+    int first = len, last = -1;
+    for (int i = 0; i < len; ++i)
+      if (dist[base + i * stride] < thresh) {
+        if (first == len) first = i;
+        last = i;
+      }
+    for (int i = 0; i < len; ++i) {
+      if (!(first < i && i < last)) inside[base + i * stride] = 0;
+    }
+  }
+}
+
+__global__ void k_fill_u8(unsigned char* p, long long n, unsigned char v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+void launch_voxel_inside(const double* dist, double thresh, int nx, int ny, int nz, unsigned char* inside,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_u8, dim3(1024), dim3(256), 0, s, inside, (long long)nx * ny * nz, (unsigned char)1);
+  hipLaunchKernelGGL(k_voxel_inside, dim3(1024), dim3(256), 0, s, dist, thresh, nx, ny, nz, inside);
+}
+
+}  // namespace bdda

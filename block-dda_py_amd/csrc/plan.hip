@@ -1,0 +1,1099 @@
+// Host side of libbdda: plans, the 5-launch Goodman FFT operator, the
+// device-resident block-COCG-rq / block-BiCGSTAB solvers (float64 and
+// mixed precision with float64 refinement) and the C ABI of include/bdda.h.
+//
+// Solver state lives on the device; per iteration only L x L Gram matrices
+// (and a few scalars) travel to the host, where the float64 coefficient
+// solves, Cholesky-QR factors and the convergence test run.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/bdda.h"
+#include "host_la.h"
+#include "internal.h"
+
+namespace bdda {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+using la::zc;
+
+#define BDDA_CHECK_LAUNCH()                                                                 \
+  do {                                                                                      \
+    hipError_t e_ = hipGetLastError();                                                      \
+    if (e_ != hipSuccess) {                                                                 \
+      set_error(std::string("kernel launch: ") + hipGetErrorString(e_));                    \
+      return BDDA_E_HIP;                                                                    \
+    }                                                                                       \
+    if (!g_err.empty() && g_err.rfind("line FFT", 0) == 0) return BDDA_E_ARG;               \
+  } while (0)
+
+// Device buffer owned by a plan.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  int ensure(size_t b) {
+    if (b <= bytes) return 0;
+    release();
+    if (hipMalloc(&p, b) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("hipMalloc of " + std::to_string(b) + " bytes failed");
+      return BDDA_E_OOM;
+    }
+    bytes = b;
+    return 0;
+  }
+  template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+// Pinned host staging buffer.
+struct HostBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~HostBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  int ensure(size_t b) {
+    if (b <= bytes) return 0;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    if (hipHostMalloc(&p, b, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("hipHostMalloc failed");
+      return BDDA_E_OOM;
+    }
+    bytes = b;
+    return 0;
+  }
+  template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+}  // namespace bdda
+
+struct bdda_plan {
+  bdda::AxisPlan ax[3];
+  long long ncell = 0;
+  bool green_ready = false;
+  bool target_ready = false;
+  bool has_pos = false;
+  long long N = 0;
+  // twiddles
+  bdda::DevBuf tw64[3], tw32[3];
+  // Green half spectrum
+  bdda::DevBuf g64, g32;
+  // target
+  bdda::DevBuf inv, pos, diag64, jpre64, diag32, jpre32;
+  // FFT intermediates
+  bdda::DevBuf T1, T2;
+  // solver workspace (blocks of 3N x L)
+  bdda::DevBuf blk[12];
+  bdda::DevBuf blk32[10];
+  bdda::DevBuf gram_part, gram_out, coef, dots_d, dots_n, obs_part, Bj64;
+  bdda::HostBuf h_gram, h_coef, h_dots;
+  // optional per-pass kernel timing (HIP events recorded around each pass)
+  bool profiling = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<int> ev_pass;  // pass id of each recorded pair (precision * 5 + pass)
+  ~bdda_plan() {
+    for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+  }
+};
+
+namespace bdda {
+
+static int check_plan(const bdda_plan* p) {
+  if (!p) {
+    set_error("null plan");
+    return BDDA_E_ARG;
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------ operator
+template <typename V> struct PrecTraits;
+template <> struct PrecTraits<double2> {
+  static const void* tw(const bdda_plan* p, int a) { return p->tw64[a].p; }
+  static const void* green(const bdda_plan* p) { return p->g64.p; }
+  static const void* diag(const bdda_plan* p) { return p->diag64.p; }
+  static const void* jpre(const bdda_plan* p) { return p->jpre64.p; }
+};
+template <> struct PrecTraits<float2> {
+  static const void* tw(const bdda_plan* p, int a) { return p->tw32[a].p; }
+  static const void* green(const bdda_plan* p) { return p->g32.p; }
+  static const void* diag(const bdda_plan* p) { return p->diag32.p; }
+  static const void* jpre(const bdda_plan* p) { return p->jpre32.p; }
+};
+
+// Column tile Lb: every line-FFT tile (m x 3 Lb complex) must fit 4096
+// values; among the admissible tiles pick the one that pads L least, then the
+// widest (longer contiguous runs per grid point).
+static int choose_lb(const bdda_plan* p, int L) {
+  const int mmax = std::max(p->ax[0].m, std::max(p->ax[1].m, p->ax[2].m));
+  int cap = 4096 / (3 * mmax);
+  cap = std::max(1, std::min(cap, 16));
+  int best = 1;
+  long long best_pad = (long long)L;
+  for (int lb = cap; lb >= 1; --lb) {
+    const long long pad = (long long)((L + lb - 1) / lb) * lb;
+    if (pad < best_pad || (pad == best_pad && lb > best)) {
+      best_pad = pad;
+      best = lb;
+    }
+  }
+  return best;
+}
+
+// Event pair around one launch when profiling is on.
+static hipEvent_t prof_event(bdda_plan* p) {
+  if (p->ev_used == p->ev_pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    p->ev_pool.push_back(e);
+  }
+  return p->ev_pool[p->ev_used++];
+}
+
+template <typename V>
+static void launch_timed(bdda_plan* p, const LinePass& lp, int pass, hipStream_t s) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (p->profiling && p->ev_used + 2 <= 4096) {
+    a = prof_event(p);
+    b = prof_event(p);
+  }
+  if (a && b) (void)hipEventRecord(a, s);
+  launch_line_pass<V>(lp, s);
+  if (a && b) {
+    (void)hipEventRecord(b, s);
+    p->ev_pass.push_back((sizeof(V) == sizeof(float2) ? 5 : 0) + pass);
+  }
+}
+
+template <typename V>
+static int ensure_fft_ws(bdda_plan* p, int L) {
+  const int Lb = choose_lb(p, L);
+  const long long nlo = (L + Lb - 1) / Lb;
+  const long long Q = 3LL * Lb;
+  const long long nx = p->ax[0].n, ny = p->ax[1].n, mz = p->ax[2].m, my = p->ax[1].m;
+  int rc = p->T1.ensure((size_t)(nlo * nx * ny * mz * Q) * sizeof(V));
+  if (rc) return rc;
+  return p->T2.ensure((size_t)(nlo * nx * my * mz * Q) * sizeof(V));
+}
+
+// Goodman convolution of L columns: z pass (scatter), y pass, fused x pass
+// (forward, Green multiply, inverse), inverse y pass, inverse z pass (gather).
+// occupied=true: S/AS are (3N, L) occupied blocks and the epilogue applies
+// jpre * (diag * S + .); otherwise (3n, L) cuboid blocks and AS = A_off S.
+template <typename V>
+static int apply_operator(bdda_plan* p, const V* S, V* AS, int L, bool occupied, hipStream_t s) {
+  const int Lb = choose_lb(p, L);
+  const int Q = 3 * Lb;
+  const int nlo = (L + Lb - 1) / Lb;
+  const int nx = p->ax[0].n, ny = p->ax[1].n, nz = p->ax[2].n;
+  const int mx = p->ax[0].m, my = p->ax[1].m, mz = p->ax[2].m;
+  int rc = ensure_fft_ws<V>(p, L);
+  if (rc) return rc;
+  g_err.clear();
+  const long long t1_lo = (long long)nx * ny * mz * Q;
+  const long long t2_lo = (long long)nx * my * mz * Q;
+  LinePass base;
+  std::memset(&base, 0, sizeof(base));
+  base.Q = Q;
+  base.L = L;
+  base.Lb = Lb;
+  base.nlo = nlo;
+  base.nz = nz;
+  base.nyz_y = ny;
+  base.inv = p->inv.as<int>();
+  base.S = S;
+  base.AS = AS;
+  base.diag = PrecTraits<V>::diag(p);
+  base.jpre = PrecTraits<V>::jpre(p);
+  base.mx = mx;
+  base.my = my;
+  base.mz = mz;
+  base.hx = p->ax[0].h;
+  base.hy = p->ax[1].h;
+  base.hz = p->ax[2].h;
+  base.green = PrecTraits<V>::green(p);
+  auto axis = [&](LinePass& lp, int a) {
+    lp.m = p->ax[a].m;
+    lp.nrad = p->ax[a].nrad;
+    for (int i = 0; i < kMaxRadix; ++i) lp.rad[i] = p->ax[a].rad[i];
+    lp.tw = PrecTraits<V>::tw(p, a);
+  };
+  // P1: z forward with scatter (points (ix, iy))
+  {
+    LinePass lp = base;
+    axis(lp, 2);
+    lp.mode = occupied ? PASS_SCATTER_Z : PASS_CUBOID_Z;
+    lp.nin = nz;
+    lp.nout = mz;
+    lp.npts = nx * ny;
+    lp.ptb = lp.npts;
+    lp.db = (long long)mz * Q;
+    lp.de = Q;
+    lp.lo_dst = t1_lo;
+    lp.dst = p->T1.p;
+    lp.order_pe = 0;
+    launch_timed<V>(p, lp, 0, s);
+    BDDA_CHECK_LAUNCH();
+  }
+  // P2: y forward (points (ix, kz)), T1 -> T2
+  {
+    LinePass lp = base;
+    axis(lp, 1);
+    lp.mode = PASS_PLAIN;
+    lp.nin = ny;
+    lp.nout = my;
+    lp.npts = nx * mz;
+    lp.ptb = mz;
+    lp.sa = (long long)ny * mz * Q;
+    lp.sb = Q;
+    lp.se = (long long)mz * Q;
+    lp.lo_src = t1_lo;
+    lp.da = (long long)my * mz * Q;
+    lp.db = Q;
+    lp.de = (long long)mz * Q;
+    lp.lo_dst = t2_lo;
+    lp.src = p->T1.p;
+    lp.dst = p->T2.p;
+    lp.order_pe = 1;
+    launch_timed<V>(p, lp, 1, s);
+    BDDA_CHECK_LAUNCH();
+  }
+  // P3: x forward * Green * x inverse (points (ky, kz)), in place on T2
+  {
+    LinePass lp = base;
+    axis(lp, 0);
+    lp.mode = PASS_CONV_X;
+    lp.nin = nx;
+    lp.nout = nx;
+    lp.npts = my * mz;
+    lp.ptb = lp.npts;
+    lp.sb = Q;
+    lp.se = (long long)my * mz * Q;
+    lp.db = Q;
+    lp.de = (long long)my * mz * Q;
+    lp.lo_src = t2_lo;
+    lp.lo_dst = t2_lo;
+    lp.src = p->T2.p;
+    lp.dst = p->T2.p;
+    lp.order_pe = 1;
+    launch_timed<V>(p, lp, 2, s);
+    BDDA_CHECK_LAUNCH();
+  }
+  // P4: y inverse (points (ix, kz)), T2 -> T1
+  {
+    LinePass lp = base;
+    axis(lp, 1);
+    lp.mode = PASS_PLAIN;
+    lp.inverse = 1;
+    lp.nin = my;
+    lp.nout = ny;
+    lp.npts = nx * mz;
+    lp.ptb = mz;
+    lp.sa = (long long)my * mz * Q;
+    lp.sb = Q;
+    lp.se = (long long)mz * Q;
+    lp.lo_src = t2_lo;
+    lp.da = (long long)ny * mz * Q;
+    lp.db = Q;
+    lp.de = (long long)mz * Q;
+    lp.lo_dst = t1_lo;
+    lp.src = p->T2.p;
+    lp.dst = p->T1.p;
+    lp.order_pe = 1;
+    launch_timed<V>(p, lp, 3, s);
+    BDDA_CHECK_LAUNCH();
+  }
+  // P5: z inverse with crop and gather/epilogue (points (ix, iy))
+  {
+    LinePass lp = base;
+    axis(lp, 2);
+    lp.mode = occupied ? PASS_GATHER_Z : PASS_CROP_Z;
+    lp.inverse = 1;
+    lp.nin = mz;
+    lp.nout = nz;
+    lp.npts = nx * ny;
+    lp.ptb = lp.npts;
+    lp.sb = (long long)mz * Q;
+    lp.se = Q;
+    lp.lo_src = t1_lo;
+    lp.src = p->T1.p;
+    lp.scale = 1.0 / ((double)mx * my * mz);
+    lp.order_pe = 0;
+    launch_timed<V>(p, lp, 4, s);
+    BDDA_CHECK_LAUNCH();
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------ solver helpers
+struct Ctx {
+  bdda_plan* p;
+  int L;
+  long long rows;
+  hipStream_t s;
+  std::vector<double>* hist;
+};
+
+static int gram_to_host(Ctx& c, const void* A, const void* B, bool conj, bool f32, std::vector<zc>& out) {
+  bdda_plan* p = c.p;
+  const int L = c.L;
+  const int ntile = (L + 63) / 64;
+  const int np = gram_parts();
+  int rc = p->gram_part.ensure((size_t)np * ntile * ntile * 4096 * sizeof(double2));
+  if (!rc) rc = p->gram_out.ensure((size_t)L * L * sizeof(double2));
+  if (!rc) rc = p->h_gram.ensure((size_t)L * L * sizeof(double2));
+  if (rc) return rc;
+  if (f32)
+    launch_gram<float2>((const float2*)A, (const float2*)B, c.rows, L, conj, p->gram_part.as<double2>(), np,
+                        p->gram_out.as<double2>(), c.s);
+  else
+    launch_gram<double2>((const double2*)A, (const double2*)B, c.rows, L, conj, p->gram_part.as<double2>(), np,
+                         p->gram_out.as<double2>(), c.s);
+  BDDA_HIP(hipGetLastError());
+  BDDA_HIP(hipMemcpyAsync(p->h_gram.p, p->gram_out.p, (size_t)L * L * sizeof(double2), hipMemcpyDeviceToHost, c.s));
+  BDDA_HIP(hipStreamSynchronize(c.s));
+  out.resize((size_t)L * L);
+  std::memcpy(out.data(), p->h_gram.p, (size_t)L * L * sizeof(double2));
+  return 0;
+}
+
+// Upload an L x L coefficient matrix into coefficient slot `slot` (pinned
+// staging; every reuse of a slot is separated by a stream synchronisation in
+// the solvers, so the host never overwrites a slot still being copied).
+static int coef_upload(Ctx& c, const std::vector<zc>& C, int slot, const double2** dev) {
+  bdda_plan* p = c.p;
+  const size_t one = (size_t)c.L * c.L * sizeof(double2);
+  int rc = p->coef.ensure(8 * one);
+  if (!rc) rc = p->h_coef.ensure(8 * one);
+  if (rc) return rc;
+  char* h = p->h_coef.as<char>() + slot * one;
+  std::memcpy(h, C.data(), one);
+  char* d = p->coef.as<char>() + slot * one;
+  BDDA_HIP(hipMemcpyAsync(d, h, one, hipMemcpyHostToDevice, c.s));
+  *dev = reinterpret_cast<const double2*>(d);
+  return 0;
+}
+
+template <typename V>
+static int dots_to_host(Ctx& c, const V* A, const V* B, zc* dot, double* nrm) {
+  bdda_plan* p = c.p;
+  const int np = dot_parts();
+  int rc = p->dots_d.ensure(np * sizeof(double2));
+  if (!rc) rc = p->dots_n.ensure(np * sizeof(double));
+  if (!rc) rc = p->h_dots.ensure(np * (sizeof(double2) + sizeof(double)));
+  if (rc) return rc;
+  launch_dot<V>(A, B, c.rows * c.L, dot ? p->dots_d.as<double2>() : nullptr, nrm ? p->dots_n.as<double>() : nullptr,
+                np, c.s);
+  BDDA_HIP(hipGetLastError());
+  double2* hd = p->h_dots.as<double2>();
+  double* hn = reinterpret_cast<double*>(hd + np);
+  if (dot) BDDA_HIP(hipMemcpyAsync(hd, p->dots_d.p, np * sizeof(double2), hipMemcpyDeviceToHost, c.s));
+  if (nrm) BDDA_HIP(hipMemcpyAsync(hn, p->dots_n.p, np * sizeof(double), hipMemcpyDeviceToHost, c.s));
+  BDDA_HIP(hipStreamSynchronize(c.s));
+  if (dot) {
+    zc s = 0.0;
+    for (int i = 0; i < np; ++i) s += zc(hd[i].x, hd[i].y);
+    *dot = s;
+  }
+  if (nrm) {
+    double s = 0.0;
+    for (int i = 0; i < np; ++i) s += hn[i];
+    *nrm = s;
+  }
+  return 0;
+}
+
+template <typename V>
+static int axpy_norm_host(Ctx& c, V* Y, const V* X, zc a, const V* Z, double* nrm2) {
+  bdda_plan* p = c.p;
+  const int np = dot_parts();
+  int rc = p->dots_n.ensure(np * sizeof(double));
+  if (!rc) rc = p->h_dots.ensure(np * (sizeof(double2) + sizeof(double)));
+  if (rc) return rc;
+  launch_axpy_norm<V>(Y, X, make_double2(a.real(), a.imag()), Z, c.rows * c.L, p->dots_n.as<double>(), np, c.s);
+  BDDA_HIP(hipGetLastError());
+  double* hn = p->h_dots.as<double>();
+  BDDA_HIP(hipMemcpyAsync(hn, p->dots_n.p, np * sizeof(double), hipMemcpyDeviceToHost, c.s));
+  BDDA_HIP(hipStreamSynchronize(c.s));
+  double s = 0.0;
+  for (int i = 0; i < np; ++i) s += hn[i];
+  *nrm2 = s;
+  return 0;
+}
+
+template <typename V>
+static int update(Ctx& c, V* Y, const V* X, zc a, const V* W1, const V* W2, zc w2c, const std::vector<zc>& C,
+                  int slot, const V* E, zc e) {
+  const double2* dC = nullptr;
+  int rc = coef_upload(c, C, slot, &dC);
+  if (rc) return rc;
+  launch_update<V>(Y, X, make_double2(a.real(), a.imag()), W1, W2, make_double2(w2c.real(), w2c.imag()), dC, E,
+                   make_double2(e.real(), e.imag()), c.rows, c.L, c.s);
+  BDDA_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename V> static constexpr bool is_f32() { return sizeof(V) == sizeof(float2); }
+
+template <typename V>
+static int op(Ctx& c, const V* S, V* AS) {
+  return apply_operator<V>(c.p, S, AS, c.L, true, c.s);
+}
+
+static int singular(const char* what) {
+  set_error(std::string("singular matrix in ") + what);
+  return BDDA_E_SINGULAR;
+}
+
+// Block-BiCGSTAB on the preconditioned system (bl_krylov.py:121-187).
+// Bj: preconditioned right-hand side; X: in/out (x_zero -> starts from 0).
+// bnorm: ||jpre B||_F used for err.  ws: 7 work blocks.
+template <typename V>
+static int solve_bicgstab(Ctx& c, const V* Bj, V* X, bool x_zero, double bnorm, double tol, int itermax, V** ws,
+                          int* it_out, double* err_out) {
+  const long long n = c.rows * c.L;
+  const size_t bytes = n * sizeof(V);
+  V *R = ws[0], *P = ws[1], *R0 = ws[2], *Vb = ws[3], *T = ws[4], *Z = ws[5], *Pn = ws[6];
+  int rc;
+  if (x_zero) {
+    BDDA_HIP(hipMemsetAsync(X, 0, bytes, c.s));
+    BDDA_HIP(hipMemcpyAsync(R, Bj, bytes, hipMemcpyDeviceToDevice, c.s));
+  } else {
+    if ((rc = op<V>(c, X, T))) return rc;
+    double dummy;
+    if ((rc = axpy_norm_host<V>(c, R, Bj, zc(-1.0), T, &dummy))) return rc;
+  }
+  BDDA_HIP(hipMemcpyAsync(P, R, bytes, hipMemcpyDeviceToDevice, c.s));
+  BDDA_HIP(hipMemcpyAsync(R0, R, bytes, hipMemcpyDeviceToDevice, c.s));
+  const int L = c.L;
+  std::vector<zc> RV, RR, RZ, alpha, beta, negA;
+  std::vector<int> piv;
+  int it = 0;
+  double err = INFINITY;
+  for (int k = 0; k < itermax; ++k) {
+    if ((rc = op<V>(c, P, Vb))) return rc;
+    if ((rc = gram_to_host(c, R0, Vb, true, is_f32<V>(), RV))) return rc;
+    if ((rc = gram_to_host(c, R0, R, true, is_f32<V>(), RR))) return rc;
+    std::vector<zc> LU = RV;
+    if (!la::lu_factor(LU, piv, L)) return singular("block-BiCGSTAB (R0^H V)");
+    alpha = RR;
+    la::lu_solve(LU, piv, L, alpha, L);
+    negA.resize(alpha.size());
+    for (size_t i = 0; i < alpha.size(); ++i) negA[i] = -alpha[i];
+    // T = R - V alpha
+    if ((rc = update<V>(c, T, R, 1.0, Vb, nullptr, 0.0, negA, 0, nullptr, 0.0))) return rc;
+    if ((rc = op<V>(c, T, Z))) return rc;
+    zc zt, zz;
+    double zn2;
+    if ((rc = dots_to_host<V>(c, Z, T, &zt, nullptr))) return rc;
+    if ((rc = dots_to_host<V>(c, Z, Z, &zz, &zn2))) return rc;
+    const zc xi = zt / zz;
+    // X = X + P alpha + xi T
+    if ((rc = update<V>(c, X, X, 1.0, P, nullptr, 0.0, alpha, 1, T, xi))) return rc;
+    // R = T - xi Z
+    double rn2;
+    if ((rc = axpy_norm_host<V>(c, R, T, -xi, Z, &rn2))) return rc;
+    err = std::sqrt(rn2) / bnorm;
+    if (c.hist) c.hist->push_back(err);
+    it = k;
+    if (err < tol) break;
+    if ((rc = gram_to_host(c, R0, Z, true, is_f32<V>(), RZ))) return rc;
+    beta = RZ;
+    for (auto& v : beta) v = -v;
+    la::lu_solve(LU, piv, L, beta, L);
+    // Pn = R + (P - xi V) beta
+    if ((rc = update<V>(c, Pn, R, 1.0, P, Vb, -xi, beta, 2, nullptr, 0.0))) return rc;
+    std::swap(P, Pn);
+  }
+  *it_out = it;
+  *err_out = err;
+  return 0;
+}
+
+// Reduced QR of W by Cholesky-QR2 in float64: Qn (device) and tau (host).
+template <typename V>
+static int cholqr2(Ctx& c, const V* W, V* Q1, V* Qn, std::vector<zc>& tau) {
+  const int L = c.L;
+  std::vector<zc> G, R1, R1i, R2, R2i;
+  int rc;
+  if ((rc = gram_to_host(c, W, W, true, is_f32<V>(), G))) return rc;
+  if (!la::cholesky_upper(G, R1, L)) return singular("block QR (Cholesky of W^H W)");
+  la::upper_inverse(R1, R1i, L);
+  if ((rc = update<V>(c, Q1, nullptr, 0.0, W, nullptr, 0.0, R1i, 3, nullptr, 0.0))) return rc;
+  if ((rc = gram_to_host(c, Q1, Q1, true, is_f32<V>(), G))) return rc;
+  if (!la::cholesky_upper(G, R2, L)) return singular("block QR (Cholesky of Q1^H Q1)");
+  la::upper_inverse(R2, R2i, L);
+  if ((rc = update<V>(c, Qn, nullptr, 0.0, Q1, nullptr, 0.0, R2i, 4, nullptr, 0.0))) return rc;
+  la::matmul(R2, false, R1, tau, L);
+  return 0;
+}
+
+// Block-COCG-rq on the preconditioned system (bl_krylov.py:58-118).
+template <typename V>
+static int solve_cocg(Ctx& c, const V* Bj, V* X, bool x_zero, double bnorm, double tol, int itermax, V** ws,
+                      int* it_out, double* err_out) {
+  const long long n = c.rows * c.L;
+  const size_t bytes = n * sizeof(V);
+  V *Q = ws[0], *S = ws[1], *AS = ws[2], *W = ws[3], *Q1 = ws[4], *Sn = ws[5], *R0 = ws[6];
+  int rc;
+  const int L = c.L;
+  if (x_zero) {
+    BDDA_HIP(hipMemsetAsync(X, 0, bytes, c.s));
+    BDDA_HIP(hipMemcpyAsync(R0, Bj, bytes, hipMemcpyDeviceToDevice, c.s));
+  } else {
+    if ((rc = op<V>(c, X, AS))) return rc;
+    double dummy;
+    if ((rc = axpy_norm_host<V>(c, R0, Bj, zc(-1.0), AS, &dummy))) return rc;
+  }
+  std::vector<zc> xi, QtQ, StAS, alpha, ax, tau, QntQn, rhs, beta, negA;
+  std::vector<int> piv;
+  if ((rc = cholqr2<V>(c, R0, Q1, Q, xi))) return rc;
+  BDDA_HIP(hipMemcpyAsync(S, Q, bytes, hipMemcpyDeviceToDevice, c.s));
+  if ((rc = gram_to_host(c, Q, Q, false, is_f32<V>(), QtQ))) return rc;
+  int it = 0;
+  double err = INFINITY;
+  for (int k = 0; k < itermax; ++k) {
+    if ((rc = op<V>(c, S, AS))) return rc;
+    if ((rc = gram_to_host(c, S, AS, false, is_f32<V>(), StAS))) return rc;
+    std::vector<zc> LU = StAS;
+    if (!la::lu_factor(LU, piv, L)) return singular("block-COCG-rq (S^T A S)");
+    alpha = QtQ;
+    la::lu_solve(LU, piv, L, alpha, L);
+    la::matmul(alpha, false, xi, ax, L);
+    // X = X + S (alpha xi)
+    if ((rc = update<V>(c, X, X, 1.0, S, nullptr, 0.0, ax, 0, nullptr, 0.0))) return rc;
+    // W = Q - AS alpha
+    negA.resize(alpha.size());
+    for (size_t i = 0; i < alpha.size(); ++i) negA[i] = -alpha[i];
+    if ((rc = update<V>(c, W, Q, 1.0, AS, nullptr, 0.0, negA, 1, nullptr, 0.0))) return rc;
+    // [Qn, tau] = qr(W); Qn written into AS (free until the next product)
+    V* Qn = AS;
+    if ((rc = cholqr2<V>(c, W, Q1, Qn, tau))) return rc;
+    std::vector<zc> xin;
+    la::matmul(tau, false, xi, xin, L);
+    xi.swap(xin);
+    err = la::frob(xi) / bnorm;
+    if (c.hist) c.hist->push_back(err);
+    it = k;
+    if (err < tol) break;
+    if ((rc = gram_to_host(c, Qn, Qn, false, is_f32<V>(), QntQn))) return rc;
+    la::matmul(tau, true, QntQn, rhs, L);  // tau^T (Qn^T Qn)
+    LU = QtQ;
+    if (!la::lu_factor(LU, piv, L)) return singular("block-COCG-rq (Q^T Q)");
+    beta = rhs;
+    la::lu_solve(LU, piv, L, beta, L);
+    // Sn = Qn + S beta ; Q <- Qn
+    if ((rc = update<V>(c, Sn, Qn, 1.0, S, nullptr, 0.0, beta, 2, nullptr, 0.0))) return rc;
+    std::swap(S, Sn);
+    BDDA_HIP(hipMemcpyAsync(Q, Qn, bytes, hipMemcpyDeviceToDevice, c.s));
+    QtQ.swap(QntQn);
+  }
+  *it_out = it;
+  *err_out = err;
+  return 0;
+}
+
+template <typename V>
+static int run_solver(Ctx& c, int solver, const V* Bj, V* X, bool x_zero, double bnorm, double tol, int itermax,
+                      V** ws, int* it, double* err) {
+  if (solver == BDDA_SOLVER_BICGSTAB) return solve_bicgstab<V>(c, Bj, X, x_zero, bnorm, tol, itermax, ws, it, err);
+  return solve_cocg<V>(c, Bj, X, x_zero, bnorm, tol, itermax, ws, it, err);
+}
+
+static int frob2(Ctx& c, const double2* A, double* out) { return dots_to_host<double2>(c, A, A, nullptr, out); }
+
+}  // namespace bdda
+
+using namespace bdda;
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char* bdda_last_error(void) { return g_err.c_str(); }
+int bdda_version(void) { return 100; }
+
+int bdda_plan_create(int nx, int ny, int nz, bdda_plan** out) {
+  g_err.clear();
+  if (!out || nx < 1 || ny < 1 || nz < 1) {
+    set_error("bdda_plan_create: lattice dimensions must be >= 1");
+    return BDDA_E_ARG;
+  }
+  std::unique_ptr<bdda_plan> p(new bdda_plan());
+  const int n[3] = {nx, ny, nz};
+  for (int a = 0; a < 3; ++a) {
+    p->ax[a].n = n[a];
+    int m = 2 * n[a];
+    // keep the reference's 2N grid when it is 5-smooth, else the smallest
+    // 5-smooth length >= 2N - 1 (still an exact linear convolution)
+    if (smallest_5smooth_at_least(m) != m) m = smallest_5smooth_at_least(2 * n[a] - 1);
+    if (m < 2) m = 2;
+    if (!factor_length(m, p->ax[a])) {
+      set_error("bdda_plan_create: cannot factor FFT length");
+      return BDDA_E_ARG;
+    }
+    if ((long long)m * 3 > 4096) {
+      set_error("bdda_plan_create: axis too long (FFT length must be <= 1365)");
+      return BDDA_E_ARG;
+    }
+    // twiddles exp(-2 pi i k / m)
+    std::vector<double2> t64(m);
+    std::vector<float2> t32(m);
+    for (int k = 0; k < m; ++k) {
+      const double ang = -2.0 * M_PI * (double)k / (double)m;
+      t64[k] = make_double2(std::cos(ang), std::sin(ang));
+      t32[k] = make_float2((float)t64[k].x, (float)t64[k].y);
+    }
+    int rc = p->tw64[a].ensure(m * sizeof(double2));
+    if (!rc) rc = p->tw32[a].ensure(m * sizeof(float2));
+    if (rc) return rc;
+    BDDA_HIP(hipMemcpy(p->tw64[a].p, t64.data(), m * sizeof(double2), hipMemcpyHostToDevice));
+    BDDA_HIP(hipMemcpy(p->tw32[a].p, t32.data(), m * sizeof(float2), hipMemcpyHostToDevice));
+  }
+  p->ncell = (long long)nx * ny * nz;
+  *out = p.release();
+  return 0;
+}
+
+int bdda_plan_destroy(bdda_plan* plan) {
+  if (plan) {
+    (void)hipDeviceSynchronize();
+    delete plan;
+  }
+  return 0;
+}
+
+int bdda_plan_fft_dims(const bdda_plan* plan, int* m_out) {
+  if (int rc = check_plan(plan)) return rc;
+  for (int a = 0; a < 3; ++a) m_out[a] = plan->ax[a].m;
+  return 0;
+}
+
+int bdda_green_init(bdda_plan* p, double lf, double k) {
+  if (int rc = check_plan(p)) return rc;
+  g_err.clear();
+  const long long M = (long long)p->ax[0].m * p->ax[1].m * p->ax[2].m;
+  const long long H = (long long)p->ax[0].h * p->ax[1].h * p->ax[2].h;
+  DevBuf full;
+  int rc = full.ensure((size_t)M * 6 * sizeof(double2));
+  if (!rc) rc = p->g64.ensure((size_t)H * 6 * sizeof(double2));
+  if (!rc) rc = p->g32.ensure((size_t)H * 6 * sizeof(float2));
+  if (rc) return rc;
+  hipStream_t s = 0;
+  launch_green_build<double2>(full.as<double2>(), p->ax, lf, k, s);
+  BDDA_HIP(hipGetLastError());
+  const int mx = p->ax[0].m, my = p->ax[1].m, mz = p->ax[2].m;
+  // in-place 3-D FFT of the 6 components, layout [x][y][z][6]
+  for (int a = 2; a >= 0; --a) {
+    LinePass lp;
+    std::memset(&lp, 0, sizeof(lp));
+    lp.mode = PASS_PLAIN;
+    lp.m = p->ax[a].m;
+    lp.nin = lp.m;
+    lp.nout = lp.m;
+    lp.nrad = p->ax[a].nrad;
+    for (int i = 0; i < kMaxRadix; ++i) lp.rad[i] = p->ax[a].rad[i];
+    lp.tw = p->tw64[a].p;
+    lp.Q = 6;
+    lp.nlo = 1;
+    lp.src = full.p;
+    lp.dst = full.p;
+    if (a == 2) {  // z lines, points (x, y)
+      lp.npts = mx * my;
+      lp.ptb = lp.npts;
+      lp.sb = lp.db = (long long)mz * 6;
+      lp.se = lp.de = 6;
+      lp.order_pe = 0;
+    } else if (a == 1) {  // y lines, points (x, z)
+      lp.npts = mx * mz;
+      lp.ptb = mz;
+      lp.sa = lp.da = (long long)my * mz * 6;
+      lp.sb = lp.db = 6;
+      lp.se = lp.de = (long long)mz * 6;
+      lp.order_pe = 1;
+    } else {  // x lines, points (y, z)
+      lp.npts = my * mz;
+      lp.ptb = lp.npts;
+      lp.sb = lp.db = 6;
+      lp.se = lp.de = (long long)my * mz * 6;
+      lp.order_pe = 1;
+    }
+    launch_line_pass<double2>(lp, s);
+    BDDA_CHECK_LAUNCH();
+  }
+  launch_green_halve<double2>(full.as<double2>(), p->g64.as<double2>(), p->ax, s);
+  launch_green_halve<float2>(full.as<double2>(), p->g32.as<float2>(), p->ax, s);
+  BDDA_HIP(hipGetLastError());
+  BDDA_HIP(hipDeviceSynchronize());
+  p->green_ready = true;
+  return 0;
+}
+
+int bdda_green_download(const bdda_plan* p, void* out_host) {
+  if (int rc = check_plan(p)) return rc;
+  if (!p->green_ready) {
+    set_error("bdda_green_download: call bdda_green_init first");
+    return BDDA_E_STATE;
+  }
+  const long long M = (long long)p->ax[0].m * p->ax[1].m * p->ax[2].m;
+  DevBuf full;
+  if (int rc = full.ensure((size_t)M * 9 * sizeof(double2))) return rc;
+  launch_green_expand(p->g64.as<double2>(), full.as<double2>(), p->ax, nullptr, 0);
+  BDDA_HIP(hipGetLastError());
+  BDDA_HIP(hipMemcpy(out_host, full.p, (size_t)M * 9 * sizeof(double2), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int bdda_green_tensor(const bdda_plan* p, double lf, double k, void* out_host) {
+  if (int rc = check_plan(p)) return rc;
+  g_err.clear();
+  const long long M = (long long)p->ax[0].m * p->ax[1].m * p->ax[2].m;
+  DevBuf g6, g9;
+  int rc = g6.ensure((size_t)M * 6 * sizeof(double2));
+  if (!rc) rc = g9.ensure((size_t)M * 9 * sizeof(double2));
+  if (rc) return rc;
+  launch_green_build<double2>(g6.as<double2>(), p->ax, lf, k, 0);
+  launch_green_sym9(g6.as<double2>(), g9.as<double2>(), M, 0);
+  BDDA_HIP(hipGetLastError());
+  BDDA_HIP(hipMemcpy(out_host, g9.p, (size_t)M * 9 * sizeof(double2), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int bdda_mvp_offdiag(bdda_plan* p, const void* P_dev, void* Y_dev, int L, int precision, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  g_err.clear();
+  if (!p->green_ready) {
+    set_error("bdda_mvp_offdiag: call bdda_green_init first");
+    return BDDA_E_STATE;
+  }
+  if (L < 1 || !P_dev || !Y_dev) {
+    set_error("bdda_mvp_offdiag: bad arguments");
+    return BDDA_E_ARG;
+  }
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (precision == BDDA_PREC_FP64)
+    return apply_operator<double2>(p, (const double2*)P_dev, (double2*)Y_dev, L, false, s);
+  // complex64 transforms on a converted copy
+  const long long n = 3 * p->ncell * (long long)L;
+  DevBuf a, b;
+  int rc = a.ensure(n * sizeof(float2));
+  if (!rc) rc = b.ensure(n * sizeof(float2));
+  if (rc) return rc;
+  launch_convert<float2, double2>(a.as<float2>(), (const double2*)P_dev, n, 1.0, s);
+  if ((rc = apply_operator<float2>(p, a.as<float2>(), b.as<float2>(), L, false, s))) return rc;
+  launch_convert<double2, float2>((double2*)Y_dev, b.as<float2>(), n, 1.0, s);
+  BDDA_HIP(hipGetLastError());
+  BDDA_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int bdda_profile_enable(bdda_plan* p, int on) {
+  if (int rc = check_plan(p)) return rc;
+  p->profiling = on != 0;
+  p->ev_used = 0;
+  p->ev_pass.clear();
+  return 0;
+}
+
+int bdda_profile_read(bdda_plan* p, double* ms_sum, int64_t* count) {
+  if (int rc = check_plan(p)) return rc;
+  g_err.clear();
+  for (int i = 0; i < 10; ++i) {
+    ms_sum[i] = 0.0;
+    count[i] = 0;
+  }
+  BDDA_HIP(hipDeviceSynchronize());
+  for (size_t i = 0; i < p->ev_pass.size(); ++i) {
+    float ms = 0.f;
+    BDDA_HIP(hipEventElapsedTime(&ms, p->ev_pool[2 * i], p->ev_pool[2 * i + 1]));
+    ms_sum[p->ev_pass[i]] += ms;
+    count[p->ev_pass[i]] += 1;
+  }
+  p->ev_used = 0;
+  p->ev_pass.clear();
+  return 0;
+}
+
+int bdda_set_target(bdda_plan* p, const int64_t* address, int64_t N, const double* pos, const void* diag,
+                    const void* jpre) {
+  if (int rc = check_plan(p)) return rc;
+  g_err.clear();
+  if (N < 1 || !address || !diag || !jpre) {
+    set_error("bdda_set_target: bad arguments");
+    return BDDA_E_ARG;
+  }
+  for (int64_t i = 0; i < N; ++i) {
+    if (address[i] < 0 || address[i] >= p->ncell || (i > 0 && address[i] <= address[i - 1])) {
+      set_error("bdda_set_target: addresses must be increasing and inside the cuboid");
+      return BDDA_E_ARG;
+    }
+  }
+  int rc = p->inv.ensure(p->ncell * sizeof(int));
+  if (!rc) rc = p->pos.ensure(N * 3 * sizeof(double));
+  if (!rc) rc = p->diag64.ensure(3 * N * sizeof(double2));
+  if (!rc) rc = p->jpre64.ensure(3 * N * sizeof(double2));
+  if (!rc) rc = p->diag32.ensure(3 * N * sizeof(float2));
+  if (!rc) rc = p->jpre32.ensure(3 * N * sizeof(float2));
+  if (rc) return rc;
+  DevBuf addr;
+  if ((rc = addr.ensure(N * sizeof(long long)))) return rc;
+  BDDA_HIP(hipMemcpy(addr.p, address, N * sizeof(long long), hipMemcpyHostToDevice));
+  launch_fill_inv(p->inv.as<int>(), p->ncell, addr.as<long long>(), N, 0);
+  if (pos) BDDA_HIP(hipMemcpy(p->pos.p, pos, N * 3 * sizeof(double), hipMemcpyHostToDevice));
+  p->has_pos = pos != nullptr;
+  BDDA_HIP(hipMemcpy(p->diag64.p, diag, 3 * N * sizeof(double2), hipMemcpyHostToDevice));
+  BDDA_HIP(hipMemcpy(p->jpre64.p, jpre, 3 * N * sizeof(double2), hipMemcpyHostToDevice));
+  launch_convert<float2, double2>(p->diag32.as<float2>(), p->diag64.as<double2>(), 3 * N, 1.0, 0);
+  launch_convert<float2, double2>(p->jpre32.as<float2>(), p->jpre64.as<double2>(), 3 * N, 1.0, 0);
+  BDDA_HIP(hipGetLastError());
+  BDDA_HIP(hipDeviceSynchronize());
+  p->N = N;
+  p->target_ready = true;
+  return 0;
+}
+
+int bdda_block_mvp(bdda_plan* p, const void* S_dev, void* AS_dev, int L, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  g_err.clear();
+  if (!p->green_ready || !p->target_ready) {
+    set_error("bdda_block_mvp: call bdda_green_init and bdda_set_target first");
+    return BDDA_E_STATE;
+  }
+  if (L < 1 || !S_dev || !AS_dev) {
+    set_error("bdda_block_mvp: bad arguments");
+    return BDDA_E_ARG;
+  }
+  return apply_operator<double2>(p, (const double2*)S_dev, (double2*)AS_dev, L, true,
+                                 reinterpret_cast<hipStream_t>(stream));
+}
+
+int bdda_solve(bdda_plan* p, int solver, int precision, const void* B_dev, const void* X0_dev, void* X_dev, int L,
+               double tol, int itermax, int refine_every, int* iter_fin, double* err_fin, double* true_res,
+               double* err_hist, int* flags_out, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  g_err.clear();
+  if (!p->green_ready || !p->target_ready) {
+    set_error("bdda_solve: call bdda_green_init and bdda_set_target first");
+    return BDDA_E_STATE;
+  }
+  if (L < 1 || L > 128 || !B_dev || !X_dev || itermax < 0 ||
+      (solver != BDDA_SOLVER_COCG_RQ && solver != BDDA_SOLVER_BICGSTAB) ||
+      (precision != BDDA_PREC_FP64 && precision != BDDA_PREC_MIXED)) {
+    set_error("bdda_solve: bad arguments (1 <= L <= 128, known solver and precision)");
+    return BDDA_E_ARG;
+  }
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  std::vector<double> hist;
+  Ctx c{p, L, 3 * p->N, s, &hist};
+  const long long n = c.rows * L;
+  const size_t b64 = n * sizeof(double2), b32 = n * sizeof(float2);
+  int rc = p->Bj64.ensure(b64);
+  for (int i = 0; i < 9 && !rc; ++i) rc = p->blk[i].ensure(b64);
+  if (rc) return rc;
+  double2* X = (double2*)X_dev;
+  double2* Bj = p->Bj64.as<double2>();
+  double2* tmp = p->blk[8].as<double2>();
+  // Bj = jpre * B, bnorm = ||Bj||_F  (bl_krylov.py:87-88)
+  launch_scale_rows<double2>(Bj, (const double2*)B_dev, p->jpre64.as<double2>(), c.rows, L, s);
+  BDDA_HIP(hipGetLastError());
+  double bn2 = 0.0;
+  if ((rc = frob2(c, Bj, &bn2))) return rc;
+  const double bnorm = std::sqrt(bn2);
+  // warm start: keep X0 only if its preconditioned residual is below ||Bj||
+  bool x_zero = true;
+  int flags = 0;
+  if (X0_dev) {
+    if ((rc = apply_operator<double2>(p, (const double2*)X0_dev, tmp, L, true, s))) return rc;
+    double r2 = 0.0;
+    if ((rc = axpy_norm_host<double2>(c, tmp, Bj, zc(-1.0), tmp, &r2))) return rc;
+    if (std::sqrt(r2) <= bnorm) {
+      x_zero = false;
+      flags |= 1;
+      if ((const void*)X0_dev != (void*)X) BDDA_HIP(hipMemcpyAsync(X, X0_dev, b64, hipMemcpyDeviceToDevice, s));
+    }
+  }
+  int it = 0;
+  double err = INFINITY;
+  if (precision == BDDA_PREC_FP64) {
+    double2* ws[7];
+    for (int i = 0; i < 7; ++i) ws[i] = p->blk[i].as<double2>();
+    if ((rc = run_solver<double2>(c, solver, Bj, X, x_zero, bnorm, tol, itermax, ws, &it, &err))) return rc;
+  } else {
+    // mixed precision: complex64 inner solves on the current true residual,
+    // float64 accumulation of X and a float64 true residual after each cycle
+    for (int i = 0; i < 9 && !rc; ++i) rc = p->blk32[i].ensure(b32);
+    if (rc) return rc;
+    float2* ws[7];
+    for (int i = 0; i < 7; ++i) ws[i] = p->blk32[i].as<float2>();
+    float2* R32 = p->blk32[7].as<float2>();
+    float2* D32 = p->blk32[8].as<float2>();
+    double2* R64 = p->blk[7].as<double2>();
+    if (x_zero) {
+      BDDA_HIP(hipMemsetAsync(X, 0, b64, s));
+      BDDA_HIP(hipMemcpyAsync(R64, Bj, b64, hipMemcpyDeviceToDevice, s));
+    } else {
+      if ((rc = apply_operator<double2>(p, X, tmp, L, true, s))) return rc;
+      double d;
+      if ((rc = axpy_norm_host<double2>(c, R64, Bj, zc(-1.0), tmp, &d))) return rc;
+    }
+    double r2;
+    if ((rc = frob2(c, R64, &r2))) return rc;
+    double res = std::sqrt(r2) / bnorm;
+    const int cycle = refine_every > 0 ? refine_every : itermax;
+    int total = 0;
+    err = res;
+    while (res >= tol && total < itermax) {
+      const double rs = std::sqrt(r2);
+      launch_convert<float2, double2>(R32, R64, n, 1.0 / rs, s);
+      BDDA_HIP(hipGetLastError());
+      double inner_tol = 0.5 * tol * bnorm / rs;
+      if (inner_tol < 2e-6) inner_tol = 2e-6;
+      std::vector<double> inner;
+      c.hist = &inner;
+      int iit = 0;
+      double ierr = 0.0;
+      const int budget = std::min(cycle, itermax - total);
+      if ((rc = run_solver<float2>(c, solver, R32, D32, true, 1.0, inner_tol, budget, ws, &iit, &ierr))) return rc;
+      c.hist = &hist;
+      for (double e : inner) hist.push_back(e * rs / bnorm);
+      total += (int)inner.size();
+      launch_add_c64_to_c128(X, D32, n, rs, s);
+      BDDA_HIP(hipGetLastError());
+      if ((rc = apply_operator<double2>(p, X, tmp, L, true, s))) return rc;
+      if ((rc = axpy_norm_host<double2>(c, R64, Bj, zc(-1.0), tmp, &r2))) return rc;
+      res = std::sqrt(r2) / bnorm;
+      if (!hist.empty()) hist.back() = res;
+      if (inner.empty()) break;
+    }
+    it = total > 0 ? total - 1 : 0;
+    err = res;
+  }
+  if (true_res) {
+    if (precision == BDDA_PREC_MIXED) {
+      *true_res = err;
+    } else {
+      if ((rc = apply_operator<double2>(p, X, tmp, L, true, s))) return rc;
+      double r2 = 0.0;
+      if ((rc = axpy_norm_host<double2>(c, tmp, Bj, zc(-1.0), tmp, &r2))) return rc;
+      *true_res = std::sqrt(r2) / bnorm;
+    }
+  }
+  BDDA_HIP(hipStreamSynchronize(s));
+  if (iter_fin) *iter_fin = it;
+  if (err_fin) *err_fin = err;
+  if (err_hist)
+    for (size_t i = 0; i < hist.size() && (int)i < std::max(itermax, 1); ++i) err_hist[i] = hist[i];
+  if (flags_out) *flags_out = flags | ((int)std::min<size_t>(hist.size(), 0x7fffff) << 8);
+  return 0;
+}
+
+int bdda_rhs(bdda_plan* p, const double* u, const void* e0, double k, int L, void* B_dev, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  g_err.clear();
+  if (!p->target_ready || !p->has_pos || L < 1 || !u || !e0 || !B_dev) {
+    set_error("bdda_rhs: bad arguments or no target");
+    return BDDA_E_ARG;
+  }
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  DevBuf du, de;
+  int rc = du.ensure(L * 3 * sizeof(double));
+  if (!rc) rc = de.ensure(L * 3 * sizeof(double2));
+  if (rc) return rc;
+  BDDA_HIP(hipMemcpy(du.p, u, L * 3 * sizeof(double), hipMemcpyHostToDevice));
+  BDDA_HIP(hipMemcpy(de.p, e0, L * 3 * sizeof(double2), hipMemcpyHostToDevice));
+  launch_rhs((double2*)B_dev, p->pos.as<double>(), du.as<double>(), de.as<double2>(), k, p->N, L, s);
+  BDDA_HIP(hipGetLastError());
+  BDDA_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int bdda_observables(bdda_plan* p, const void* X_dev, int L, const double* geo, const void* epsfac, double k,
+                     void* out_host, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  g_err.clear();
+  if (!p->target_ready || !p->has_pos || L < 1 || !X_dev || !geo || !epsfac || !out_host) {
+    set_error("bdda_observables: bad arguments or no target");
+    return BDDA_E_ARG;
+  }
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const int np = obs_parts();
+  DevBuf dg, dfac;
+  int rc = dg.ensure(L * 24 * sizeof(double));
+  if (!rc) rc = dfac.ensure(3 * sizeof(double2));
+  if (!rc) rc = p->obs_part.ensure((size_t)np * L * 5 * sizeof(double2));
+  if (rc) return rc;
+  BDDA_HIP(hipMemcpy(dg.p, geo, L * 24 * sizeof(double), hipMemcpyHostToDevice));
+  BDDA_HIP(hipMemcpy(dfac.p, epsfac, 3 * sizeof(double2), hipMemcpyHostToDevice));
+  launch_observables((const double2*)X_dev, p->pos.as<double>(), dg.as<double>(), dfac.as<double2>(), k, p->N, L,
+                     p->obs_part.as<double2>(), np, s);
+  BDDA_HIP(hipGetLastError());
+  std::vector<double2> part((size_t)np * L * 5);
+  BDDA_HIP(hipMemcpyAsync(part.data(), p->obs_part.p, part.size() * sizeof(double2), hipMemcpyDeviceToHost, s));
+  BDDA_HIP(hipStreamSynchronize(s));
+  double2* out = reinterpret_cast<double2*>(out_host);
+  for (int l = 0; l < L; ++l)
+    for (int t = 0; t < 5; ++t) {
+      double sx = 0.0, sy = 0.0;
+      for (int w = 0; w < np; ++w) {
+        sx += part[((size_t)w * L + l) * 5 + t].x;
+        sy += part[((size_t)w * L + l) * 5 + t].y;
+      }
+      out[l * 5 + t] = make_double2(sx, sy);
+    }
+  return 0;
+}
+
+int bdda_nearest_distance(const double* grid, int64_t n, const double* surf, int64_t P, double* dist_host) {
+  g_err.clear();
+  if (!grid || !surf || !dist_host || n < 1 || P < 1) {
+    set_error("bdda_nearest_distance: bad arguments");
+    return BDDA_E_ARG;
+  }
+  DevBuf dgrid, dsurf, ddist;
+  int rc = dgrid.ensure(n * 3 * sizeof(double));
+  if (!rc) rc = dsurf.ensure(P * 3 * sizeof(double));
+  if (!rc) rc = ddist.ensure(n * sizeof(double));
+  if (rc) return rc;
+  BDDA_HIP(hipMemcpy(dgrid.p, grid, n * 3 * sizeof(double), hipMemcpyHostToDevice));
+  BDDA_HIP(hipMemcpy(dsurf.p, surf, P * 3 * sizeof(double), hipMemcpyHostToDevice));
+  launch_voxel_dist(dgrid.as<double>(), n, dsurf.as<double>(), P, ddist.as<double>(), 0);
+  BDDA_HIP(hipGetLastError());
+  BDDA_HIP(hipMemcpy(dist_host, ddist.p, n * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int bdda_inside_mask(const double* dist, int nx, int ny, int nz, double thresh, uint8_t* inside_host) {
+  g_err.clear();
+  if (!dist || !inside_host || nx < 1 || ny < 1 || nz < 1) {
+    set_error("bdda_inside_mask: bad arguments");
+    return BDDA_E_ARG;
+  }
+  const long long n = (long long)nx * ny * nz;
+  DevBuf ddist, dins;
+  int rc = ddist.ensure(n * sizeof(double));
+  if (!rc) rc = dins.ensure(n);
+  if (rc) return rc;
+  BDDA_HIP(hipMemcpy(ddist.p, dist, n * sizeof(double), hipMemcpyHostToDevice));
+  launch_voxel_inside(ddist.as<double>(), thresh, nx, ny, nz, dins.as<unsigned char>(), 0);
+  BDDA_HIP(hipGetLastError());
+  BDDA_HIP(hipMemcpy(inside_host, dins.p, n, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,157 @@
+/*
+ * bdda.h — C ABI of libbdda, the MI355X (gfx950) block-DDA solver.
+ *
+ * Drop-in boundary for the reference block-DDA_Py v0.3.0 solve path
+ * (mvp_fft -> bl_krylov -> bl_dda.scatterer).  Every entry point names the
+ * reference interface it replaces.  Plain pointers and sizes only:
+ *
+ *   *_dev   device pointers (e.g. torch CUDA tensors' data_ptr())
+ *   *_host  host pointers
+ *   complex values are interleaved (re, im) doubles ("c128") in the
+ *   reference's layouts: Krylov blocks (3N, L) row-major with row = 3m + c,
+ *   cuboid blocks (3n, L) with row = 3*(ix*Ny*Nz + iy*Nz + iz) + c.
+ *
+ * Every function returns BDDA_OK (0) or an error code; bdda_last_error()
+ * gives the message of the last failure in the calling thread.  A plan is
+ * not re-entrant: calls on one plan must not overlap.
+ */
+#ifndef BDDA_H_
+#define BDDA_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bdda_plan bdda_plan;
+
+enum {
+  BDDA_OK = 0,
+  BDDA_E_ARG = 1,      /* invalid argument (the reference raises ValueError)      */
+  BDDA_E_HIP = 2,      /* HIP runtime failure                                       */
+  BDDA_E_SINGULAR = 3, /* singular L x L system (numpy.linalg.LinAlgError)          */
+  BDDA_E_OOM = 4,      /* device allocation failed                                  */
+  BDDA_E_STATE = 5     /* call order (e.g. solve before green_init / set_target)    */
+};
+
+enum { BDDA_SOLVER_COCG_RQ = 0, BDDA_SOLVER_BICGSTAB = 1 };
+enum { BDDA_PREC_FP64 = 0, BDDA_PREC_MIXED = 1 };
+
+/* Message of the last failed call in this thread ("" if none). */
+const char* bdda_last_error(void);
+/* ABI version (major * 100 + minor). */
+int bdda_version(void);
+
+/* Plan over an nx x ny x nz cuboid lattice: chooses the convolution FFT
+ * lengths m >= 2n - 1 per axis and owns every device buffer of the path.
+ * Replaces the shape/allocation part of fft_init (mvp_fft/mvp_fft.py:97-119). */
+int bdda_plan_create(int nx, int ny, int nz, bdda_plan** out);
+int bdda_plan_destroy(bdda_plan* plan);
+/* FFT lengths per axis (3 ints). */
+int bdda_plan_fft_dims(const bdda_plan* plan, int* m_out);
+
+/* Green tensor A'(r) on the doubled grid and its 3-D FFT, kept on device in
+ * complex128 and complex64 (6 unique components, half spectrum).
+ * Replaces build_interaction_tensor + fft_init (mvp_fft/mvp_fft.py:13-119). */
+int bdda_green_init(bdda_plan* plan, double lattice_lf, double k);
+/* The full (m0, m1, m2, 3, 3) complex128 spectrum, i.e. what fft_init
+ * returns when m = 2n on every axis (mvp_fft/mvp_fft.py:119). */
+int bdda_green_download(const bdda_plan* plan, void* out_host);
+
+/* The real-space (m0, m1, m2, 3, 3) complex128 tensor A'(r) before the FFT,
+ * built on the device; equals build_interaction_tensor's array when m = 2n
+ * on every axis.  Replaces build_interaction_tensor (mvp_fft/mvp_fft.py:13-94). */
+int bdda_green_tensor(const bdda_plan* plan, double lattice_lf, double k, void* out_host);
+
+/* Off-diagonal product on the full cuboid, Y = A_off P, for L columns.
+ * P_dev, Y_dev: (3n, L) complex128.  precision: BDDA_PREC_FP64 or
+ * BDDA_PREC_MIXED (complex64 transforms).
+ * Replaces fft_mvp (mvp_fft/mvp_fft.py:122-173). */
+int bdda_mvp_offdiag(bdda_plan* plan, const void* P_dev, void* Y_dev, int L, int precision, void* stream);
+
+/* Occupied sites and the Jacobi-scaled diagonal.
+ * address_host: (N) cuboid addresses (ascending, as np.where gives them);
+ * pos_host: (N, 3) float64 positions (NULL: no positions; bdda_rhs and
+ * bdda_observables then refuse); diag_host, jpre_host: (3N) complex128
+ * diag(A) and 1/diag(A).
+ * Replaces _build_scatter_index and the jpre set-up of the solvers
+ * (bl_krylov/bl_krylov.py:5-24,84,146). */
+int bdda_set_target(bdda_plan* plan, const int64_t* address_host, int64_t N, const double* pos_host,
+                    const void* diag_host, const void* jpre_host);
+
+/* AS = jpre * (diag * S + A_off S) for L columns; S_dev, AS_dev: (3N, L)
+ * complex128.  Replaces _block_mvp (bl_krylov/bl_krylov.py:27-55). */
+int bdda_block_mvp(bdda_plan* plan, const void* S_dev, void* AS_dev, int L, void* stream);
+
+/* Block-Krylov solve of A X = B with left Jacobi preconditioning.
+ *   solver        BDDA_SOLVER_COCG_RQ | BDDA_SOLVER_BICGSTAB
+ *   precision     BDDA_PREC_FP64: the reference algorithm in complex128;
+ *                 BDDA_PREC_MIXED: complex64 operator and vector updates,
+ *                 float64 L x L solves / block QR, true float64 residual and
+ *                 refinement every refine_every iterations.
+ *   B_dev         (3N, L) complex128 right-hand side
+ *   X0_dev        (3N, L) complex128 initial guess or NULL (zero start); a
+ *                 guess whose preconditioned residual exceeds ||jpre B|| is
+ *                 discarded (zero start)
+ *   X_dev         (3N, L) complex128 solution
+ *   iter_fin      0-based index of the last iteration; err_fin the last
+ *                 recurrence (FP64) or true (mixed) relative residual,
+ *                 Frobenius over the block, / ||jpre B||
+ *   true_res      recomputed ||jpre (B - A X)|| / ||jpre B|| (may be NULL)
+ *   err_hist      per-iteration err (host, itermax doubles, may be NULL)
+ *   flags_out     bit 0: the initial guess was used (may be NULL)
+ * Non-convergence is not an error: err_fin >= tol, as in the reference.
+ * Replaces bl_cocg_rq_jacobi_mvp_fft and bl_bicgstab_jacobi_mvp_fft
+ * (bl_krylov/bl_krylov.py:58-187). */
+int bdda_solve(bdda_plan* plan, int solver, int precision, const void* B_dev, const void* X0_dev, void* X_dev,
+               int L, double tol, int itermax, int refine_every, int* iter_fin, double* err_fin,
+               double* true_res, double* err_hist, int* flags_out, void* stream);
+
+/* Per-pass kernel timing of the FFT operator with HIP events recorded on
+ * the launch stream around every pass (z, y, x-convolution, inverse y,
+ * inverse z; complex128 then complex64), for measurement only.
+ * bdda_profile_read synchronises the device and returns, for the 10
+ * (precision, pass) slots, the summed milliseconds and launch counts since
+ * the last read, then resets them (at most 2048 launches are recorded). */
+int bdda_profile_enable(bdda_plan* plan, int on);
+int bdda_profile_read(bdda_plan* plan, double* ms_sum, int64_t* count);
+
+/* Right-hand side B[3m+c][l] = exp(i k u_l . r_m) e0[l][c] into B_dev
+ * (3N, L) complex128; u_host (L, 3) float64, e0_host (L, 3) complex128.
+ * Replaces DiscreteDipoles.set_interaction_matrix :153-161 (bl_dda/scatterer.py). */
+int bdda_rhs(bdda_plan* plan, const double* u_host, const void* e0_host, double k, int L, void* B_dev,
+             void* stream);
+
+/* Per-orientation observable sums from X_dev (3N, L) complex128.
+ * geo_host (L, 24) float64: u_fw, th_fw, ph_fw, u_bk, th_bk, ph_bk, e0
+ * (3 complex); epsfac_host (3) complex128: 4 pi / ((eps_r - 1) v) per axis.
+ * out_host (L, 5) complex128:
+ *   [0] sum P.th_fw conj(e_fw)       [1] sum P.ph_fw (-i) conj(e_fw)
+ *   [2] sum P.th_bk conj(e_bk)       [3] sum P.ph_bk (-i) conj(e_bk)
+ *   [4] (sum Im(P conj(E)), sum Im(P conj(E_inc)))
+ * Replaces compute_C_abs / compute_C_ext / compute_PCAS_observable_S_fw /
+ * compute_OCBS_observable_S_bk (bl_dda/scatterer.py:187-228). */
+int bdda_observables(bdda_plan* plan, const void* X_dev, int L, const double* geo_host, const void* epsfac_host,
+                     double k, void* out_host, void* stream);
+
+/* Nearest surface-point distance of every lattice point, bit-exact with
+ * the KD-tree query: sqrt(min_j ((dx*dx + dy*dy) + dz*dz)).
+ * grid_host (n, 3), surf_host (P, 3) float64; dist_host (n) float64.
+ * Replaces find_nearest_distance_from_the_GRE_surf
+ * (shape_model/gaussian_ellipsoid.py:182-192). */
+int bdda_nearest_distance(const double* grid_host, int64_t n, const double* surf_host, int64_t P,
+                          double* dist_host);
+
+/* Three-axis ray-cast inside test: near = dist < thresh; a voxel is inside
+ * along an axis when a near voxel lies strictly before and strictly after
+ * it; AND over the three axes.  dist_host (nx*ny*nz) float64, thresh =
+ * distance_factor * lattice_lf; inside_host (nx*ny*nz) bytes (0/1).
+ * Replaces extract_lattice_address_in_GRE_volume
+ * (shape_model/gaussian_ellipsoid.py:195-237). */
+int bdda_inside_mask(const double* dist_host, int nx, int ny, int nz, double thresh, uint8_t* inside_host);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BDDA_H_ */

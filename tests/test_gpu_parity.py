@@ -1,0 +1,249 @@
+"""GPU parity of the HIP path (through the C ABI) against the float64 oracle.
+
+Tolerances: float64 path 1e-12 (operator) / 1e-8 (solver iterates, which
+amplify rounding by the iteration count) relative; S(0) and S_bk within 1e-6
+relative in float64 mode and 1e-4 in mixed mode (the required bars); the GRE
+voxel mask and distances bit-exact.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():
+    pytest.skip("no GPU", allow_module_level=True)
+
+from oracle import bl_krylov as okr  # noqa: E402
+from oracle import gre as ogre  # noqa: E402
+from oracle import mvp_fft as omvp  # noqa: E402
+from oracle import orient, scatterer as osc  # noqa: E402
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+# --------------------------------------------------------------- operator
+@pytest.mark.parametrize("n,L", [((4, 5, 6), 3), ((7, 5, 11), 9), ((8, 8, 8), 16), ((3, 9, 4), 1)])
+def test_fft_mvp_matches_oracle(n, L):
+    say(f"fft_mvp n={n} L={L}")
+    from mvp_fft import mvp_fft as pm
+    lf, k = 0.031, 12.7
+    rng = np.random.default_rng(0)
+    ncell = int(np.prod(n))
+    P = rng.standard_normal((3 * ncell, L)) + 1j * rng.standard_normal((3 * ncell, L))
+    ref = omvp.fft_mvp(n, 3, omvp.fft_init(n, 3, lf, k), P)
+    At = pm.fft_init(n, 3, lf, k)
+    Y = pm.fft_mvp(n, 3, At, P)
+    assert Y.shape == ref.shape and Y.dtype == np.complex128
+    assert rel(Y, ref) < 1e-12
+    Ym = pm.fft_mvp(n, 3, At, P, precision="mixed")
+    assert rel(Ym, ref) < 2e-6
+    y1 = pm.fft_mvp(n, 3, At, P[:, 0])
+    assert y1.shape == (3 * ncell,) and rel(y1, ref[:, 0]) < 1e-12
+
+
+def test_green_spectrum_and_tensor_match_oracle():
+    say("green spectrum")
+    from mvp_fft import mvp_fft as pm
+    n, lf, k = (5, 6, 4), 0.04, 9.1
+    At = pm.fft_init(n, 3, lf, k)
+    assert At.shape == (10, 12, 8, 3, 3)
+    g = np.asarray(At)
+    ref = omvp.fft_init(n, 3, lf, k)
+    assert rel(g, ref) < 1e-13
+    t = pm.build_interaction_tensor(n, lf, k)
+    tref = omvp.interaction_tensor(n, lf, k)
+    assert rel(t, tref) < 1e-14
+    assert np.array_equal(t == 0, tref == 0)  # identical zero planes
+    with pytest.raises(ValueError):
+        pm.fft_init(n, 2, lf, k)
+
+
+def test_green_non_smooth_grid_still_exact_convolution():
+    say("non-smooth grid")
+    from mvp_fft import mvp_fft as pm
+    n = (17, 7, 13)  # 2N = 34, 14, 26 are not 5-smooth
+    At = pm.fft_init(n, 3, 0.02, 15.0)
+    assert At.shape[:3] == (36, 15, 25)
+    with pytest.raises(ValueError):
+        np.asarray(At)
+
+
+# ----------------------------------------------------------------- solver
+def _problem(r=0.07, bc=1.2, ab=1.1, beta=0.0, m_p=(1.5 + 0.01j,) * 3, L=6, wl=0.35, mm=1.2, seed=0):
+    name, n, lf, grid, is_in, rve, _ = ogre.build_gre_geometry(r, bc, ab, beta, seed=seed)
+    return name, n, lf, grid, is_in, np.asarray(m_p), orient.euler_run_dda(L, seed=seed + 1), wl, mm
+
+
+def _oracle_solve(prob, solver, tol, itermax):
+    name, n, lf, grid, is_in, m_p, eu, wl, mm = prob
+    tg = osc.Target(n, lf, grid, is_in, m_p)
+    inc = osc.Incident(wl, mm, eu)
+    return osc.solve(tg, inc, solver, tol, itermax)
+
+
+def _gpu_dd(prob, solver="bicgstab", tol=1e-2, itermax=25, precision="fp64", **kw):
+    from bl_dda.scatterer import DiscreteDipoles, IncidentField, Target
+    name, n, lf, grid, is_in, m_p, eu, wl, mm = prob
+    dd = DiscreteDipoles(Target(name, n, lf, grid, is_in, m_p), IncidentField(wl, mm, eu))
+    dd.verbose = False
+    dd.solver, dd.tol, dd.itermax, dd.precision = solver, tol, itermax, precision
+    for a, v in kw.items():
+        setattr(dd, a, v)
+    dd.set_interaction_matrix()
+    dd.solve_matrix_equation()
+    return dd
+
+
+@pytest.mark.parametrize("solver", ["cocg_rq", "bicgstab"])
+@pytest.mark.parametrize("L", [6, 70])
+def test_solver_iterates_and_observables_match_oracle(solver, L):
+    say(f"solver {solver} L={L}")
+    prob = _problem(L=L)
+    ref = _oracle_solve(prob, "cocg" if solver == "cocg_rq" else "bicgstab", 1e-8, 60)
+    dd = _gpu_dd(prob, solver, 1e-8, 60)
+    assert dd.iter_fin == ref["iter_fin"]
+    np.testing.assert_allclose(dd.err_history, ref["history"], rtol=1e-6)
+    assert rel(dd.X, ref["X"]) < 1e-9
+    st, sp = dd.compute_PCAS_observable_S_fw()
+    sb = dd.compute_OCBS_observable_S_bk()
+    for a, b in [(st, ref["S_fw_theta"]), (sp, ref["S_fw_phi"]), (sb, ref["S_bk"])]:
+        assert np.max(np.abs(a - b) / np.abs(b)) < 1e-6
+    np.testing.assert_allclose(dd.compute_C_ext(), ref["C_ext"], rtol=1e-6)
+    np.testing.assert_allclose(dd.compute_C_abs(), ref["C_abs"], rtol=1e-6)
+    assert dd.true_residual < 1e-7
+
+
+def test_block_mvp_and_krylov_dropins_match_oracle():
+    say("block_mvp drop-in")
+    from bl_krylov import bl_krylov as pk
+    from mvp_fft import mvp_fft as pm
+    name, n, lf, grid, is_in, m_p, eu, wl, mm = _problem(L=4)
+    tg = osc.Target(n, lf, grid, is_in, m_p)
+    inc = osc.Incident(wl, mm, eu)
+    eps_r, alpha, diag = osc.polarizability(tg, inc)
+    E_inc, B = osc.rhs(tg, inc)
+    jpre = 1.0 / diag
+    At_o = omvp.fft_init(n, 3, lf, inc.k)
+    At_p = pm.fft_init(n, 3, lf, inc.k)
+    idx = pk._build_scatter_index(tg.address, 3)
+    rng = np.random.default_rng(3)
+    S = rng.standard_normal(B.shape) + 1j * rng.standard_normal(B.shape)
+    a = pk._block_mvp(n, 3, At_p, idx, diag, S, jpre)
+    b = okr.block_mvp(n, 3, At_o, idx, diag, S, jpre)
+    assert rel(a, b) < 1e-12
+    X, it, err = pk.bl_bicgstab_jacobi_mvp_fft(n, 3, tg.address, At_p, diag, B, 1e-6, 30, verbose=False)
+    Xo, ito, erro, _ = okr.bl_bicgstab(n, 3, tg.address, At_o, diag, B, 1e-6, 30)
+    assert it == ito and abs(err - erro) < 1e-6 * erro and rel(X, Xo) < 1e-9
+    X, it, err = pk.bl_cocg_rq_jacobi_mvp_fft(n, 3, tg.address, At_p, diag, B, 1e-6, 30, verbose=False)
+    Xo, ito, erro, _ = okr.bl_cocg_rq(n, 3, tg.address, At_o, diag, B, 1e-6, 30)
+    assert it == ito and abs(err - erro) < 1e-6 * erro and rel(X, Xo) < 1e-9
+
+
+def test_notebook_run_on_gpu(golden_nb):
+    """test_dda.ipynb: sphere r_v = 0.2 um, 17^3 lattice, 10 orientations."""
+    say("notebook")
+    from bl_dda.scatterer import DiscreteDipoles, IncidentField, Target
+    from shape_model.gaussian_ellipsoid import gaussian_ellipsoid_shape_model as G
+    nb = golden_nb
+    g = G(float(nb["r_v_base"]), 1, 1, 0.0)
+    pts, _ = g.compute_r_points_on_GRE(np.random.default_rng(int(nb["shape_seed"])))
+    _, n, grid = g.create_cuboid_lattice_that_encloses_GRE_shape(pts)
+    dist = g.find_nearest_distance_from_the_GRE_surf(grid, pts)
+    is_in = g.extract_lattice_address_in_GRE_volume(g.lattice_lf, g.distance_factor, n, dist)
+    assert n.tolist() == [17, 17, 17]
+    tg = Target(g.name, n, g.lattice_lf, grid, is_in, nb["m_p_xyz"])
+    dd = DiscreteDipoles(tg, IncidentField(float(nb["wl_0"]), float(nb["m_m"]), orient.euler_test_dda()))
+    assert dd.dpl == float(nb["dpl"])
+    dd.verbose = False
+    dd.set_interaction_matrix()
+    dd.solve_matrix_equation()
+    assert dd.converge and dd.iter_fin == int(nb["iter_fin"])
+    assert ["%.4f" % e for e in dd.err_history] == ["%.4f" % e for e in nb["err_printed"]]
+    np.testing.assert_allclose(dd.compute_OCBS_observable_S_bk(), nb["s_bk"], rtol=2e-6, atol=1e-8)
+
+
+def test_mixed_precision_meets_fp64_oracle():
+    say("mixed precision")
+    prob = _problem(r=0.09, L=8, m_p=(1.55 + 0.001j, 1.6 + 0.001j, 1.5 + 0.001j))
+    ref = _oracle_solve(prob, "bicgstab", 1e-10, 80)
+    for solver in ["bicgstab", "cocg_rq"]:
+        dd = _gpu_dd(prob, solver, 1e-5, 80, precision="mixed", refine_every=20)
+        assert dd.converge and dd.true_residual < 1e-5
+        st, sp = dd.compute_PCAS_observable_S_fw()
+        sb = dd.compute_OCBS_observable_S_bk()
+        for a, b in [(st, ref["S_fw_theta"]), (sp, ref["S_fw_phi"]), (sb, ref["S_bk"])]:
+            assert np.max(np.abs(a - b) / np.abs(b)) < 1e-4
+
+
+def test_warm_start_and_fallback():
+    say("warm start")
+    prob = _problem(L=5)
+    ref = _oracle_solve(prob, "bicgstab", 1e-9, 60)
+    good = ref["X"] * (1 + 1e-4)
+    dd = _gpu_dd(prob, "bicgstab", 1e-5, 60, X0=good)
+    assert dd.warm_start_used and dd.converge and dd.iter_fin <= 1
+    bad = ref["X"] * 50.0
+    dd2 = _gpu_dd(prob, "bicgstab", 1e-5, 60, X0=bad)
+    dd0 = _gpu_dd(prob, "bicgstab", 1e-5, 60)
+    assert not dd2.warm_start_used and dd2.iter_fin == dd0.iter_fin
+    assert rel(dd2.X, dd0.X) < 1e-12
+
+
+def test_nonconvergence_is_reported_not_raised():
+    say("non-convergence")
+    prob = _problem(L=3)
+    dd = _gpu_dd(prob, "bicgstab", 1e-30, 2)
+    assert not dd.converge and dd.iter_fin == 1 and dd.err_fin > 1e-30
+    assert np.all(dd.P == 0)
+    dd0 = _gpu_dd(prob, "cocg_rq", 1e-2, 0)
+    assert not dd0.converge and dd0.err_fin == float("inf")
+
+
+def test_single_dipole():
+    say("single dipole")
+    from bl_dda.scatterer import DiscreteDipoles, IncidentField, Target
+    grid = np.zeros((1, 3))
+    tg = Target("one", np.array([1, 1, 1]), 0.02, grid, np.array([True]), np.array([1.6 + 0.1j] * 3))
+    dd = DiscreteDipoles(tg, IncidentField(0.5, 1.1, orient.euler_run_dda(3, seed=4)))
+    dd.verbose = False
+    dd.tol = 1e-12
+    dd.set_interaction_matrix()
+    dd.solve_matrix_equation()
+    a = dd.alpha_E[0, 0]
+    np.testing.assert_allclose(dd.compute_PCAS_observable_S_fw()[0], dd.k ** 2 * a * np.ones(3), rtol=1e-13)
+
+
+def test_padded_cuboid_is_the_same_physics():
+    say("padding")
+    prob = _problem(L=4)
+    a = _gpu_dd(prob, "bicgstab", 1e-8, 60)
+    b = _gpu_dd(prob, "bicgstab", 1e-8, 60, pad_to=(16, 16, 16))
+    assert tuple(b.Au_til.n) == (16, 16, 16)
+    assert a.iter_fin == b.iter_fin and rel(b.X, a.X) < 1e-10
+
+
+# -------------------------------------------------------------- voxeliser
+@pytest.mark.parametrize("beta", [0.0, 0.2])
+def test_voxeliser_bit_exact(beta):
+    say(f"voxeliser beta={beta}")
+    from shape_model.gaussian_ellipsoid import gaussian_ellipsoid_shape_model as G
+    g = G(0.15, 1.3, 1.2, beta)
+    pts, _ = g.compute_r_points_on_GRE(np.random.default_rng(11))
+    _, n, grid = g.create_cuboid_lattice_that_encloses_GRE_shape(pts)
+    d_gpu = g.find_nearest_distance_from_the_GRE_surf(grid, pts)
+    d_ref = ogre.nearest_distance(grid, pts)
+    assert np.array_equal(d_gpu, d_ref)
+    m_gpu = g.extract_lattice_address_in_GRE_volume(g.lattice_lf, g.distance_factor, n, d_gpu)
+    m_ref = ogre.inside_mask(g.lattice_lf, g.distance_factor, n, d_ref)
+    assert np.array_equal(m_gpu, m_ref)
