@@ -45,6 +45,7 @@ enum PassMode : int {
   PASS_GATHER_Z = 3,    // T1 -> inverse z FFT -> diag + Jacobi epilogue -> (3N,L)
   PASS_CUBOID_Z = 4,    // cuboid (3n,L) -> z FFT -> T1
   PASS_CROP_Z = 5,      // T1 -> inverse z FFT -> cuboid (3n,L) (fft_mvp)
+  PASS_PLAIN_INV = 6,   // PASS_PLAIN with the inverse transform (register kernels)
 };
 
 struct LinePass {
@@ -76,13 +77,15 @@ struct LinePass {
   const void* jpre;     // (3N) complex
   double scale;         // 1/M for the inverse transform
   // Green multiply
-  const void* green;    // (hx, hy, hz, 6) half spectrum
+  const void* green;    // half spectrum, [(ay*hz + az)*hx + ax][6]
   int hx, hy, hz;
   int mx, my, mz;
 };
 
 template <typename V>
 void launch_line_pass(const LinePass& lp, hipStream_t s);
+// Route every length through the runtime-radix kernel (tests compare both).
+void set_force_generic_fft(bool on);
 template <typename V>
 void launch_green_build(V* out, const AxisPlan* ax, double lf, double k, hipStream_t s);
 template <typename V>

@@ -11,6 +11,8 @@
 //   T1  [lo][ix][iy][kz][c][li]   (Nx, Ny, Mz, 3, Lb)  after the z pass
 //   T2  [lo][ix][ky][kz][c][li]   (Nx, My, Mz, 3, Lb)  after the y pass
 // The x pass (forward x, 3x3 Green multiply, inverse x) runs in place on T2.
+#include <algorithm>
+
 #include "internal.h"
 
 namespace bdda {
@@ -119,28 +121,75 @@ template <bool INV, typename V> struct Dft<5, INV, V> {
   }
 };
 
-// One Stockham pass of radix R over QQ independent columns of length M.
-// Requires M*QQ <= VPT*nthr so that every thread stages at most VPT values.
+// 16-point DFT as 4 x 4: four DFT4 over stride-4 subsequences, twiddles
+// W16^(j2*k1), four DFT4 across; X[k1 + 4 k2].
+template <bool INV, typename V> struct Dft<16, INV, V> {
+  __device__ __forceinline__ static void run(V* a) {
+    using T = typename RealOf<V>::T;
+    const T c1 = (T)0.92387953251128675613, s1 = (T)0.38268343236508977173, h = (T)0.70710678118654752440;
+    V t[4][4];
+#pragma unroll
+    for (int j2 = 0; j2 < 4; ++j2) {
+      V u[4] = {a[j2], a[j2 + 4], a[j2 + 8], a[j2 + 12]};
+      Dft<4, INV, V>::run(u);
+#pragma unroll
+      for (int k1 = 0; k1 < 4; ++k1) t[j2][k1] = u[k1];
+    }
+    // W16^m = cos(2 pi m/16) -+ i sin(2 pi m/16)
+    auto tw = [&](V v, T c, T s) {  // v * (c - i s) forward, v * (c + i s) inverse
+      return INV ? cx<V>(v.x * c - v.y * s, v.y * c + v.x * s) : cx<V>(v.x * c + v.y * s, v.y * c - v.x * s);
+    };
+    t[1][1] = tw(t[1][1], c1, s1);
+    t[1][2] = tw(t[1][2], h, h);
+    t[1][3] = tw(t[1][3], s1, c1);
+    t[2][1] = tw(t[2][1], h, h);
+    t[2][2] = rotq<INV>(t[2][2]);
+    t[2][3] = tw(t[2][3], -h, h);
+    t[3][1] = tw(t[3][1], s1, c1);
+    t[3][2] = tw(t[3][2], -h, h);
+    t[3][3] = tw(t[3][3], -c1, -s1);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      V u[4] = {t[0][k1], t[1][k1], t[2][k1], t[3][k1]};
+      Dft<4, INV, V>::run(u);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) a[k1 + 4 * k2] = u[k2];
+    }
+  }
+};
+
+// Thread mapping of a tile [m][QQ] (QQ = P*Q columns): thread t owns column
+// qq = t % QQ and the element rows rt, rt + TPC, rt + 2 TPC, ... with
+// rt = t / QQ and TPC = blockDim / QQ threads per column (TPC = ceil(m/VPT)).
+// Lanes of a wavefront therefore walk consecutive columns, i.e. consecutive
+// LDS words and consecutive global addresses.
+struct TileMap {
+  int QQ, TPC, qq, rt;
+};
+
+// One Stockham pass of radix R: every column has m/R butterflies; this thread
+// does butterflies j = rt + i*TPC.  Twiddles come from the LDS copy of the
+// length-m table.  NB*TPC >= m/R because TPC = ceil(m/VPT).
 template <int R, bool INV, int VPT, typename V>
-__device__ __forceinline__ void stockham_pass(V* buf, int M, int QQ, int Ns, const V* __restrict__ tw,
-                                              int tid, int nthr) {
+__device__ __forceinline__ void stockham_pass(V* buf, const V* tws, int M, int Ns, const TileMap& tm) {
   constexpr int NB = (VPT + R - 1) / R;
   const int nb = M / R;
-  const int total = nb * QQ;
   const int ts = M / (Ns * R);
+  const int QQ = tm.QQ;
   V v[NB][R];
+  int jj[NB];
 #pragma unroll
   for (int i = 0; i < NB; ++i) {
-    const int b = tid + i * nthr;
-    if (b < total) {
-      const int q = b % QQ, j = b / QQ;
+    const int j = tm.rt + i * tm.TPC;
+    jj[i] = j;
+    if (j < nb) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[i][r] = buf[(j + r * nb) * QQ + q];
+      for (int r = 0; r < R; ++r) v[i][r] = buf[(j + r * nb) * QQ + tm.qq];
       const int jm = j % Ns;
       if (jm != 0) {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          V w = tw[jm * r * ts];
+          V w = tws[jm * r * ts];
           if (INV) w.y = -w.y;
           v[i][r] = cmul(v[i][r], w);
         }
@@ -151,29 +200,27 @@ __device__ __forceinline__ void stockham_pass(V* buf, int M, int QQ, int Ns, con
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < NB; ++i) {
-    const int b = tid + i * nthr;
-    if (b < total) {
-      const int q = b % QQ, j = b / QQ;
-      const int jm = j % Ns;
-      const int base = (j / Ns) * Ns * R + jm;
+    const int j = jj[i];
+    if (j < nb) {
+      const int base = (j / Ns) * Ns * R + j % Ns;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[(base + r * Ns) * QQ + q] = v[i][r];
+      for (int r = 0; r < R; ++r) buf[(base + r * Ns) * QQ + tm.qq] = v[i][r];
     }
   }
   __syncthreads();
 }
 
 template <bool INV, int VPT, typename V>
-__device__ void lds_fft(V* buf, const LinePass& lp, int QQ, const V* __restrict__ tw, int tid, int nthr) {
+__device__ void lds_fft(V* buf, const V* tws, const LinePass& lp, const TileMap& tm) {
   int Ns = 1;
   for (int p = 0; p < lp.nrad; ++p) {
     const int R = lp.rad[p];
     switch (R) {
-      case 8: stockham_pass<8, INV, VPT>(buf, lp.m, QQ, Ns, tw, tid, nthr); break;
-      case 4: stockham_pass<4, INV, VPT>(buf, lp.m, QQ, Ns, tw, tid, nthr); break;
-      case 2: stockham_pass<2, INV, VPT>(buf, lp.m, QQ, Ns, tw, tid, nthr); break;
-      case 3: stockham_pass<3, INV, VPT>(buf, lp.m, QQ, Ns, tw, tid, nthr); break;
-      default: stockham_pass<5, INV, VPT>(buf, lp.m, QQ, Ns, tw, tid, nthr); break;
+      case 8: stockham_pass<8, INV, VPT>(buf, tws, lp.m, Ns, tm); break;
+      case 4: stockham_pass<4, INV, VPT>(buf, tws, lp.m, Ns, tm); break;
+      case 2: stockham_pass<2, INV, VPT>(buf, tws, lp.m, Ns, tm); break;
+      case 3: stockham_pass<3, INV, VPT>(buf, tws, lp.m, Ns, tm); break;
+      default: stockham_pass<5, INV, VPT>(buf, tws, lp.m, Ns, tm); break;
     }
     Ns *= R;
   }
@@ -195,7 +242,7 @@ __device__ void green_multiply(V* buf, const LinePass& lp, int pt0, int np_here,
     if (ax > lp.mx - ax) { ax = lp.mx - ax; sx = -1; }
     if (ay > lp.my - ay) { ay = lp.my - ay; sy = -1; }
     if (az > lp.mz - az) { az = lp.mz - az; sz = -1; }
-    const V* g = G + ((long long)(ax * lp.hy + ay) * lp.hz + az) * 6;
+    const V* g = G + (((long long)ay * lp.hz + az) * lp.hx + ax) * 6;
     const V gxx = g[0], gyy = g[3], gzz = g[5];
     const V gxy = cscale(g[1], sx * sy), gxz = cscale(g[2], sx * sz), gyz = cscale(g[4], sy * sz);
     V* row = buf + qx * QQ + p * Q;
@@ -212,48 +259,66 @@ __device__ void green_multiply(V* buf, const LinePass& lp, int pt0, int np_here,
 template <typename V, int VPT, int MODE>
 __global__ void __launch_bounds__(512) k_line_pass(LinePass lp) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int M = lp.m, P = lp.P, Q = lp.Q, QQ = P * Q;
   V* buf = reinterpret_cast<V*>(smem_raw);
+  V* tws = buf + M * QQ;
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int P = lp.P, Q = lp.Q, QQ = P * Q, M = lp.m;
+  TileMap tm;
+  tm.QQ = QQ;
+  tm.TPC = nthr / QQ;
+  tm.qq = tid % QQ;
+  tm.rt = tid / QQ;
+  const int p = tm.qq / Q, q = tm.qq % Q;
   const int pt0 = blockIdx.x * P;
   const int lo = blockIdx.y;
   const int np_here = min(P, lp.npts - pt0);
-  const int tot = M * QQ;
+  const bool active = tm.rt < tm.TPC && p < np_here;
+  const int pt = pt0 + p;
   const V zero = cx<V>(0, 0);
   const V* __restrict__ src = reinterpret_cast<const V*>(lp.src) + (long long)lo * lp.lo_src;
-  const V* __restrict__ tw = reinterpret_cast<const V*>(lp.tw);
+  for (int i = tid; i < M; i += nthr) tws[i] = reinterpret_cast<const V*>(lp.tw)[i];
 
-  // ---- load (zero-padded to m) -------------------------------------------
-  for (int idx = tid; idx < tot; idx += nthr) {
-    const int q = idx % Q;
-    const int t = idx / Q;
-    int p, e;
-    if (lp.order_pe) { p = t % P; e = t / P; } else { e = t % M; p = t / M; }
-    V v = zero;
-    if (e < lp.nin && p < np_here) {
-      const int pt = pt0 + p;
-      if (MODE == PASS_SCATTER_Z || MODE == PASS_CUBOID_Z) {
+  // ---- load: every global load of this thread is issued before any LDS store
+  V v[VPT];
+  if (MODE == PASS_SCATTER_Z || MODE == PASS_CUBOID_Z) {
+    const int c = q / lp.Lb, l = lo * lp.Lb + q % lp.Lb;
+    long long mrow[VPT];
+#pragma unroll
+    for (int i = 0; i < VPT; ++i) {
+      const int e = tm.rt + i * tm.TPC;
+      mrow[i] = -1;
+      if (active && e < lp.nin && l < lp.L) {
         const long long cell = (long long)pt * lp.nz + e;
-        const long long m = (MODE == PASS_SCATTER_Z) ? (long long)lp.inv[cell] : cell;
-        const int c = q / lp.Lb, l = lo * lp.Lb + q % lp.Lb;
-        if (m >= 0 && l < lp.L) v = reinterpret_cast<const V*>(lp.S)[(3 * m + c) * lp.L + l];
-      } else {
-        const int a = pt / lp.ptb, b = pt % lp.ptb;
-        v = src[a * lp.sa + b * lp.sb + e * lp.se + q];
+        mrow[i] = (MODE == PASS_SCATTER_Z) ? (long long)lp.inv[cell] : cell;
       }
     }
-    buf[e * QQ + p * Q + q] = v;
+    const V* __restrict__ S = reinterpret_cast<const V*>(lp.S);
+#pragma unroll
+    for (int i = 0; i < VPT; ++i) v[i] = mrow[i] >= 0 ? S[(3 * mrow[i] + c) * lp.L + l] : zero;
+  } else {
+    const int a = pt / lp.ptb, b = pt % lp.ptb;
+    const V* __restrict__ base = src + a * lp.sa + b * lp.sb + q;
+#pragma unroll
+    for (int i = 0; i < VPT; ++i) {
+      const int e = tm.rt + i * tm.TPC;
+      v[i] = (active && e < lp.nin) ? base[e * lp.se] : zero;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < VPT; ++i) {
+    const int e = tm.rt + i * tm.TPC;
+    if (tm.rt < tm.TPC && e < M) buf[e * QQ + tm.qq] = v[i];
   }
   __syncthreads();
 
   // ---- transform ------------------------------------------------------------
   if (MODE == PASS_GATHER_Z || MODE == PASS_CROP_Z || (MODE == PASS_PLAIN && lp.inverse)) {
-    lds_fft<true, VPT>(buf, lp, QQ, tw, tid, nthr);
+    lds_fft<true, VPT>(buf, tws, lp, tm);
   } else {
-    lds_fft<false, VPT>(buf, lp, QQ, tw, tid, nthr);
+    lds_fft<false, VPT>(buf, tws, lp, tm);
     if (MODE == PASS_CONV_X) {
       green_multiply(buf, lp, pt0, np_here, QQ, tid, nthr);
-      lds_fft<true, VPT>(buf, lp, QQ, tw, tid, nthr);
+      lds_fft<true, VPT>(buf, tws, lp, tm);
     }
   }
 
@@ -261,46 +326,262 @@ __global__ void __launch_bounds__(512) k_line_pass(LinePass lp) {
   if (MODE == PASS_GATHER_Z || MODE == PASS_CROP_Z) {
     using T = typename RealOf<V>::T;
     const T sc = (T)lp.scale;
+    const int c = q / lp.Lb, l = lo * lp.Lb + q % lp.Lb;
+    if (!active || l >= lp.L) return;
     V* __restrict__ out = reinterpret_cast<V*>(lp.AS);
-    const V* __restrict__ S = reinterpret_cast<const V*>(lp.S);
-    const V* __restrict__ dg = reinterpret_cast<const V*>(lp.diag);
-    const V* __restrict__ jp = reinterpret_cast<const V*>(lp.jpre);
-    for (int idx = tid; idx < tot; idx += nthr) {
-      const int q = idx % Q;
-      const int t = idx / Q;
-      const int e = t % M, p = t / M;
-      if (e >= lp.nout || p >= np_here) continue;
-      const int c = q / lp.Lb, l = lo * lp.Lb + q % lp.Lb;
-      if (l >= lp.L) continue;
-      const long long cell = (long long)(pt0 + p) * lp.nz + e;
-      const V y = cscale(buf[e * QQ + p * Q + q], sc);
-      if (MODE == PASS_GATHER_Z) {
-        const long long m = lp.inv[cell];
-        if (m < 0) continue;
-        const long long row = 3 * m + c;
-        const V d = cmul(dg[row], S[row * lp.L + l]);
-        out[row * lp.L + l] = cmul(jp[row], cadd(d, y));
-      } else {
-        out[(3 * cell + c) * lp.L + l] = y;
+    long long row[VPT];
+#pragma unroll
+    for (int i = 0; i < VPT; ++i) {
+      const int e = tm.rt + i * tm.TPC;
+      row[i] = -1;
+      if (e < lp.nout) {
+        const long long cell = (long long)pt * lp.nz + e;
+        const long long m = (MODE == PASS_GATHER_Z) ? (long long)lp.inv[cell] : cell;
+        if (m >= 0) row[i] = 3 * m + c;
+      }
+    }
+    if (MODE == PASS_GATHER_Z) {
+      const V* __restrict__ S = reinterpret_cast<const V*>(lp.S);
+      const V* __restrict__ dg = reinterpret_cast<const V*>(lp.diag);
+      const V* __restrict__ jp = reinterpret_cast<const V*>(lp.jpre);
+      V sv[VPT], dv[VPT], jv[VPT];
+#pragma unroll
+      for (int i = 0; i < VPT; ++i) {
+        if (row[i] >= 0) {
+          sv[i] = S[row[i] * lp.L + l];
+          dv[i] = dg[row[i]];
+          jv[i] = jp[row[i]];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < VPT; ++i) {
+        if (row[i] >= 0) {
+          const int e = tm.rt + i * tm.TPC;
+          const V y = cscale(buf[e * QQ + tm.qq], sc);
+          out[row[i] * lp.L + l] = cmul(jv[i], cadd(cmul(dv[i], sv[i]), y));
+        }
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < VPT; ++i) {
+        if (row[i] >= 0) {
+          const int e = tm.rt + i * tm.TPC;
+          out[row[i] * lp.L + l] = cscale(buf[e * QQ + tm.qq], sc);
+        }
       }
     }
     return;
   }
+  if (!active) return;
   V* __restrict__ dst = reinterpret_cast<V*>(lp.dst) + (long long)lo * lp.lo_dst;
-  for (int idx = tid; idx < tot; idx += nthr) {
-    const int q = idx % Q;
-    const int t = idx / Q;
-    int p, e;
-    if (lp.order_pe) { p = t % P; e = t / P; } else { e = t % M; p = t / M; }
-    if (e >= lp.nout || p >= np_here) continue;
-    const int pt = pt0 + p;
-    const int a = pt / lp.ptb, b = pt % lp.ptb;
-    dst[a * lp.da + b * lp.db + e * lp.de + q] = buf[e * QQ + p * Q + q];
+  const int a = pt / lp.ptb, b = pt % lp.ptb;
+  V* __restrict__ obase = dst + a * lp.da + b * lp.db + q;
+#pragma unroll
+  for (int i = 0; i < VPT; ++i) {
+    const int e = tm.rt + i * tm.TPC;
+    if (e < lp.nout) obase[e * lp.de] = buf[e * QQ + tm.qq];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Two-step register FFT for m = R1 * R2 (R1, R2 in {2, 3, 4, 5, 8, 16}).
+// Step 1: thread (qq, j2) holds x[j1*R2 + j2] (j1 < R1) in registers, does a
+// DFT_R1, multiplies by W_m^(j2*k1) and writes LDS[k1*R2 + j2].  Step 2:
+// thread (qq, k1) reads LDS[k1*R2 + j2] (j2 < R2), does a DFT_R2 and holds
+// X[k1 + R1*k2].  One LDS exchange per transform; the inputs come straight
+// from global memory (or LDS) and the outputs go straight to global memory
+// (or LDS), so a pass costs two LDS trips instead of 2 per radix.
+template <int R1, int R2, bool INV, typename V>
+__device__ __forceinline__ void reg2_step1(V* a, const V* tws, int j2) {
+  Dft<R1, INV, V>::run(a);
+  if (j2 != 0) {
+#pragma unroll
+    for (int k1 = 1; k1 < R1; ++k1) {
+      V w = tws[j2 * k1];
+      if (INV) w.y = -w.y;
+      a[k1] = cmul(a[k1], w);
+    }
+  }
+}
+
+template <typename V, int R1, int R2, int MODE>
+__global__ void __launch_bounds__(512) k_reg2(LinePass lp) {
+  constexpr int M = R1 * R2;
+  constexpr int RMAX = R1 > R2 ? R1 : R2;
+  constexpr bool INV_PASS = (MODE == PASS_GATHER_Z || MODE == PASS_CROP_Z || MODE == PASS_PLAIN_INV);
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int P = lp.P, Q = lp.Q, QQ = P * Q;
+  V* buf = reinterpret_cast<V*>(smem_raw);
+  V* tws = buf + M * QQ;
+  V* gls = tws + M;  // Green tensor of the tile's points: [p][qx][6]
+  const int tid = threadIdx.x;
+  const int qq = tid % QQ, w = tid / QQ;
+  const int p = qq / Q, q = qq % Q;
+  const int pt0 = blockIdx.x * P;
+  const int lo = blockIdx.y;
+  const int np_here = min(P, lp.npts - pt0);
+  const bool active = p < np_here;
+  const int pt = pt0 + p;
+  const V zero = cx<V>(0, 0);
+  for (int i = tid; i < M; i += blockDim.x) tws[i] = reinterpret_cast<const V*>(lp.tw)[i];
+  if (MODE == PASS_CONV_X) {
+    // half spectrum stored x-innermost: [(ay*hz + az)*hx + ax][6]
+    const V* __restrict__ G = reinterpret_cast<const V*>(lp.green);
+    for (int i = tid; i < np_here * M * 6; i += blockDim.x) {
+      const int comp = i % 6, qx = (i / 6) % M, pp = i / (6 * M);
+      const int ptg = pt0 + pp;
+      const int ky = ptg / lp.mz, kz = ptg % lp.mz;
+      int ax = qx, ay = ky, az = kz;
+      int sgn = 0;
+      if (ax > lp.mx - ax) { ax = lp.mx - ax; sgn ^= (comp == 1 || comp == 2); }
+      if (ay > lp.my - ay) { ay = lp.my - ay; sgn ^= (comp == 1 || comp == 4); }
+      if (az > lp.mz - az) { az = lp.mz - az; sgn ^= (comp == 2 || comp == 4); }
+      V g = G[(((long long)ay * lp.hz + az) * lp.hx + ax) * 6 + comp];
+      if (sgn) g = cx<V>(-g.x, -g.y);
+      gls[i] = g;
+    }
+  }
+
+  // ---- step 1 (forward or inverse by mode), input from global ---------------
+  V a[RMAX];
+  const bool s1 = w < R2;
+  const int j2 = w;
+  {
+    const V* __restrict__ src = reinterpret_cast<const V*>(lp.src) + (long long)lo * lp.lo_src;
+    if (MODE == PASS_SCATTER_Z || MODE == PASS_CUBOID_Z) {
+      const int c = q / lp.Lb, l = lo * lp.Lb + q % lp.Lb;
+      long long mrow[R1];
+#pragma unroll
+      for (int j1 = 0; j1 < R1; ++j1) {
+        const int e = j1 * R2 + j2;
+        mrow[j1] = -1;
+        if (s1 && active && e < lp.nin && l < lp.L) {
+          const long long cell = (long long)pt * lp.nz + e;
+          mrow[j1] = (MODE == PASS_SCATTER_Z) ? (long long)lp.inv[cell] : cell;
+        }
+      }
+      const V* __restrict__ S = reinterpret_cast<const V*>(lp.S);
+#pragma unroll
+      for (int j1 = 0; j1 < R1; ++j1) a[j1] = mrow[j1] >= 0 ? S[(3 * mrow[j1] + c) * lp.L + l] : zero;
+    } else {
+      const int ga = pt / lp.ptb, gb = pt % lp.ptb;
+      const V* __restrict__ base = src + ga * lp.sa + gb * lp.sb + q;
+#pragma unroll
+      for (int j1 = 0; j1 < R1; ++j1) {
+        const int e = j1 * R2 + j2;
+        a[j1] = (s1 && active && e < lp.nin) ? base[e * lp.se] : zero;
+      }
+    }
+  }
+  __syncthreads();  // twiddles and Green tile staged
+  if (s1) {
+    reg2_step1<R1, R2, INV_PASS>(a, tws, j2);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) buf[(k1 * R2 + j2) * QQ + qq] = a[k1];
+  }
+  __syncthreads();
+
+  // ---- step 2 -----------------------------------------------------------------
+  const bool s2 = w < R1;
+  const int k1 = w;
+  if (s2) {
+#pragma unroll
+    for (int jj = 0; jj < R2; ++jj) a[jj] = buf[(k1 * R2 + jj) * QQ + qq];
+    Dft<R2, INV_PASS, V>::run(a);
+  }
+  if (MODE == PASS_CONV_X) {
+    // X[k1 + R1 k2] -> LDS natural order, 3x3 multiply, inverse transform
+    __syncthreads();
+    if (s2) {
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) buf[(k1 + R1 * k2) * QQ + qq] = a[k2];
+    }
+    __syncthreads();
+    const int Lb = lp.Lb;
+    for (int it = tid; it < np_here * Lb * M; it += blockDim.x) {
+      const int li = it % Lb, kx = (it / Lb) % M, pp = it / (Lb * M);
+      const V* g = gls + (pp * M + kx) * 6;
+      V* row = buf + kx * QQ + pp * Q;
+      const V x0 = row[li], x1 = row[Lb + li], x2 = row[2 * Lb + li];
+      row[li] = cadd(cadd(cmul(g[0], x0), cmul(g[1], x1)), cmul(g[2], x2));
+      row[Lb + li] = cadd(cadd(cmul(g[1], x0), cmul(g[3], x1)), cmul(g[4], x2));
+      row[2 * Lb + li] = cadd(cadd(cmul(g[2], x0), cmul(g[4], x1)), cmul(g[5], x2));
+    }
+    __syncthreads();
+    // inverse step 1 from LDS natural order
+    if (s1) {
+#pragma unroll
+      for (int j1 = 0; j1 < R1; ++j1) a[j1] = buf[(j1 * R2 + j2) * QQ + qq];
+    }
+    __syncthreads();
+    if (s1) {
+      reg2_step1<R1, R2, true>(a, tws, j2);
+#pragma unroll
+      for (int kk = 0; kk < R1; ++kk) buf[(kk * R2 + j2) * QQ + qq] = a[kk];
+    }
+    __syncthreads();
+    if (s2) {
+#pragma unroll
+      for (int jj = 0; jj < R2; ++jj) a[jj] = buf[(k1 * R2 + jj) * QQ + qq];
+      Dft<R2, true, V>::run(a);
+    }
+  }
+
+  // ---- store X[k1 + R1 k2], k < nout --------------------------------------------
+  if (!s2 || !active) return;
+  if (MODE == PASS_GATHER_Z || MODE == PASS_CROP_Z) {
+    using T = typename RealOf<V>::T;
+    const T sc = (T)lp.scale;
+    const int c = q / lp.Lb, l = lo * lp.Lb + q % lp.Lb;
+    if (l >= lp.L) return;
+    V* __restrict__ out = reinterpret_cast<V*>(lp.AS);
+    long long row[R2];
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) {
+      const int e = k1 + R1 * k2;
+      row[k2] = -1;
+      if (e < lp.nout) {
+        const long long cell = (long long)pt * lp.nz + e;
+        const long long m = (MODE == PASS_GATHER_Z) ? (long long)lp.inv[cell] : cell;
+        if (m >= 0) row[k2] = 3 * m + c;
+      }
+    }
+    if (MODE == PASS_GATHER_Z) {
+      const V* __restrict__ S = reinterpret_cast<const V*>(lp.S);
+      const V* __restrict__ dg = reinterpret_cast<const V*>(lp.diag);
+      const V* __restrict__ jp = reinterpret_cast<const V*>(lp.jpre);
+      V sv[R2], dv[R2], jv[R2];
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) {
+        if (row[k2] >= 0) {
+          sv[k2] = S[row[k2] * lp.L + l];
+          dv[k2] = dg[row[k2]];
+          jv[k2] = jp[row[k2]];
+        }
+      }
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2)
+        if (row[k2] >= 0) out[row[k2] * lp.L + l] = cmul(jv[k2], cadd(cmul(dv[k2], sv[k2]), cscale(a[k2], sc)));
+    } else {
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2)
+        if (row[k2] >= 0) out[row[k2] * lp.L + l] = cscale(a[k2], sc);
+    }
+    return;
+  }
+  V* __restrict__ dst = reinterpret_cast<V*>(lp.dst) + (long long)lo * lp.lo_dst;
+  const int ga = pt / lp.ptb, gb = pt % lp.ptb;
+  V* __restrict__ obase = dst + ga * lp.da + gb * lp.db + q;
+#pragma unroll
+  for (int k2 = 0; k2 < R2; ++k2) {
+    const int e = k1 + R1 * k2;
+    if (e < lp.nout) obase[e * lp.de] = a[k2];
   }
 }
 
 constexpr int kVPT = 8;
 constexpr int kTile = 4096;  // complex values per workgroup tile
+static bool g_force_generic = false;
 
 template <typename V, int MODE>
 static void launch_mode(const LinePass& lp, dim3 grid, int nthr, size_t lds, hipStream_t s) {
@@ -313,22 +594,78 @@ static void launch_mode(const LinePass& lp, dim3 grid, int nthr, size_t lds, hip
   hipLaunchKernelGGL((k_line_pass<V, kVPT, MODE>), grid, dim3(nthr), lds, s, lp);
 }
 
-template <typename V>
-void launch_line_pass(const LinePass& lp0, hipStream_t s) {
-  LinePass lp = lp0;
-  const int MQ = lp.m * lp.Q;
-  int P = kTile / MQ;
+template <typename V, int R1, int R2, int MODE>
+static void launch_reg2_mode(const LinePass& lp, dim3 grid, int nthr, size_t lds, hipStream_t s) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_reg2<V, R1, R2, MODE>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k_reg2<V, R1, R2, MODE>), grid, dim3(nthr), lds, s, lp);
+}
+
+template <typename V, int R1, int R2>
+static void launch_reg2(LinePass lp, hipStream_t s) {
+  constexpr int M = R1 * R2, RMAX = R1 > R2 ? R1 : R2;
+  const bool conv = lp.mode == PASS_CONV_X;
+  int P = std::min(512 / (lp.Q * RMAX), 4096 / (M * lp.Q));
   if (P < 1) P = 1;
   if (P > lp.npts) P = lp.npts;
   lp.P = P;
-  const int tot = MQ * P;
-  int nthr = ((tot + kVPT - 1) / kVPT + 63) / 64 * 64;
-  if (nthr < 64) nthr = 64;
-  if (nthr > 512 || tot > kVPT * nthr) {
+  const int nthr = P * lp.Q * RMAX;
+  const size_t lds = ((size_t)M * P * lp.Q + M + (conv ? (size_t)P * M * 6 : 0)) * sizeof(V);
+  dim3 grid((lp.npts + P - 1) / P, lp.nlo > 0 ? lp.nlo : 1);
+  const int mode = (lp.mode == PASS_PLAIN && lp.inverse) ? PASS_PLAIN_INV : lp.mode;
+  switch (mode) {
+    case PASS_PLAIN: launch_reg2_mode<V, R1, R2, PASS_PLAIN>(lp, grid, nthr, lds, s); break;
+    case PASS_PLAIN_INV: launch_reg2_mode<V, R1, R2, PASS_PLAIN_INV>(lp, grid, nthr, lds, s); break;
+    case PASS_SCATTER_Z: launch_reg2_mode<V, R1, R2, PASS_SCATTER_Z>(lp, grid, nthr, lds, s); break;
+    case PASS_CONV_X: launch_reg2_mode<V, R1, R2, PASS_CONV_X>(lp, grid, nthr, lds, s); break;
+    case PASS_GATHER_Z: launch_reg2_mode<V, R1, R2, PASS_GATHER_Z>(lp, grid, nthr, lds, s); break;
+    case PASS_CUBOID_Z: launch_reg2_mode<V, R1, R2, PASS_CUBOID_Z>(lp, grid, nthr, lds, s); break;
+    default: launch_reg2_mode<V, R1, R2, PASS_CROP_Z>(lp, grid, nthr, lds, s); break;
+  }
+}
+
+// Lengths with a two-step register kernel (m = R1 * R2); others use the
+// runtime-radix Stockham kernel.
+template <typename V>
+static bool try_reg2(const LinePass& lp, hipStream_t s) {
+  if (lp.Q * 16 > 512 && lp.m >= 128) return false;
+  switch (lp.m) {
+    case 16: launch_reg2<V, 4, 4>(lp, s); return true;
+    case 24: launch_reg2<V, 8, 3>(lp, s); return true;
+    case 32: launch_reg2<V, 8, 4>(lp, s); return true;
+    case 40: launch_reg2<V, 8, 5>(lp, s); return true;
+    case 48: launch_reg2<V, 16, 3>(lp, s); return true;
+    case 64: launch_reg2<V, 8, 8>(lp, s); return true;
+    case 80: launch_reg2<V, 16, 5>(lp, s); return true;
+    case 128: launch_reg2<V, 16, 8>(lp, s); return true;
+    case 256: launch_reg2<V, 16, 16>(lp, s); return true;
+    default: return false;
+  }
+}
+
+template <typename V>
+void launch_line_pass(const LinePass& lp0, hipStream_t s) {
+  LinePass lp = lp0;
+  if (lp.m * lp.Q > kTile) {
     set_error("line FFT tile too large (m*Q must be <= 4096)");
     return;
   }
-  const size_t lds = (size_t)tot * sizeof(V);
+  if (!g_force_generic && try_reg2<V>(lp, s)) return;
+  const int TPC = (lp.m + kVPT - 1) / kVPT;
+  if (lp.Q * TPC > 512) {
+    set_error("line FFT tile too large (m*Q must be <= 4096)");
+    return;
+  }
+  int P = std::min(kTile / (lp.m * lp.Q), 512 / (lp.Q * TPC));
+  if (P < 1) P = 1;
+  if (P > lp.npts) P = lp.npts;
+  lp.P = P;
+  const int nthr = P * lp.Q * TPC;
+  const size_t lds = ((size_t)lp.m * P * lp.Q + lp.m) * sizeof(V);
   dim3 grid((lp.npts + P - 1) / P, lp.nlo > 0 ? lp.nlo : 1);
   switch (lp.mode) {
     case PASS_PLAIN: launch_mode<V, PASS_PLAIN>(lp, grid, nthr, lds, s); break;
@@ -339,6 +676,8 @@ void launch_line_pass(const LinePass& lp0, hipStream_t s) {
     default: launch_mode<V, PASS_CROP_Z>(lp, grid, nthr, lds, s); break;
   }
 }
+
+void set_force_generic_fft(bool on) { g_force_generic = on; }
 
 template void launch_line_pass<double2>(const LinePass&, hipStream_t);
 template void launch_line_pass<float2>(const LinePass&, hipStream_t);
@@ -393,9 +732,10 @@ __global__ void k_green_halve(const double2* full, V* half, int mx, int my, int 
   const long long total = (long long)hx * hy * hz * 6;
   for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
        id += (long long)gridDim.x * blockDim.x) {
+    // half spectrum stored x-innermost: [(qy*hz + qz)*hx + qx][6]
     const int c = id % 6;
     const long long t = id / 6;
-    const int qz = t % hz, qy = (t / hz) % hy, qx = t / ((long long)hz * hy);
+    const int qx = t % hx, qz = (t / hx) % hz, qy = t / ((long long)hx * hz);
     const double2 g = full[(((long long)qx * my + qy) * mz + qz) * 6 + c];
     V v;
     v.x = g.x;
@@ -416,7 +756,7 @@ __global__ void k_green_expand(const double2* half, double2* full, int mx, int m
     if (ax > mx - ax) { ax = mx - ax; s[0] = -1; }
     if (ay > my - ay) { ay = my - ay; s[1] = -1; }
     if (az > mz - az) { az = mz - az; s[2] = -1; }
-    const double2* g = half + (((long long)ax * hy + ay) * hz + az) * 6;
+    const double2* g = half + (((long long)ay * hz + az) * hx + ax) * 6;
     for (int a = 0; a < 3; ++a)
       for (int b = 0; b < 3; ++b) {
         const double sg = a == b ? 1.0 : s[a] * s[b];

@@ -629,6 +629,15 @@ extern "C" {
 const char* bdda_last_error(void) { return g_err.c_str(); }
 int bdda_version(void) { return 100; }
 
+int bdda_set_fft_path(int path) {
+  if (path != 0 && path != 1) {
+    set_error("bdda_set_fft_path: path must be 0 (auto) or 1 (runtime-radix only)");
+    return BDDA_E_ARG;
+  }
+  set_force_generic_fft(path == 1);
+  return 0;
+}
+
 int bdda_plan_create(int nx, int ny, int nz, bdda_plan** out) {
   g_err.clear();
   if (!out || nx < 1 || ny < 1 || nz < 1) {

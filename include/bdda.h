@@ -43,6 +43,12 @@ const char* bdda_last_error(void);
 /* ABI version (major * 100 + minor). */
 int bdda_version(void);
 
+/* FFT kernel selection for the whole process: 0 (default) uses the
+ * two-step register kernels for lengths m = R1*R2 (R1, R2 in {2,3,4,5,8,16})
+ * and the runtime-radix LDS Stockham kernels otherwise; 1 forces the
+ * runtime-radix kernels everywhere (used to cross-check the two). */
+int bdda_set_fft_path(int path);
+
 /* Plan over an nx x ny x nz cuboid lattice: chooses the convolution FFT
  * lengths m >= 2n - 1 per axis and owns every device buffer of the path.
  * Replaces the shape/allocation part of fft_init (mvp_fft/mvp_fft.py:97-119). */

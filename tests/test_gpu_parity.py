@@ -52,6 +52,32 @@ def test_fft_mvp_matches_oracle(n, L):
     assert y1.shape == (3 * ncell,) and rel(y1, ref[:, 0]) < 1e-12
 
 
+@pytest.mark.parametrize("n", [(8, 12, 20), (16, 24, 32), (40, 64, 3), (128, 2, 3)])
+def test_register_and_stockham_kernels_agree(n):
+    """Every register-kernel length (16 ... 256) against the runtime-radix
+    kernel and the oracle, both precisions."""
+    say(f"kernel paths n={n}")
+    from bdda_amd import _lib
+    from mvp_fft import mvp_fft as pm
+    lib = _lib.load()
+    lf, k = 0.029, 10.3
+    rng = np.random.default_rng(1)
+    ncell = int(np.prod(n))
+    P = rng.standard_normal((3 * ncell, 5)) + 1j * rng.standard_normal((3 * ncell, 5))
+    ref = omvp.fft_mvp(n, 3, omvp.fft_init(n, 3, lf, k), P)
+    At = pm.fft_init(n, 3, lf, k)
+    out = {}
+    try:
+        for path in (0, 1):
+            _lib.check(lib.bdda_set_fft_path(path))
+            out[path] = (pm.fft_mvp(n, 3, At, P), pm.fft_mvp(n, 3, At, P, precision="mixed"))
+    finally:
+        lib.bdda_set_fft_path(0)
+    for path in (0, 1):
+        assert rel(out[path][0], ref) < 1e-12
+        assert rel(out[path][1], ref) < 2e-6
+
+
 def test_green_spectrum_and_tensor_match_oracle():
     say("green spectrum")
     from mvp_fft import mvp_fft as pm
