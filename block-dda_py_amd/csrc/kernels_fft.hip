@@ -417,15 +417,27 @@ __global__ void __launch_bounds__(512) k_reg2(LinePass lp) {
   const int tid = threadIdx.x;
   const int qq = tid % QQ, w = tid / QQ;
   const int p = qq / Q, q = qq % Q;
-  const int pt0 = blockIdx.x * P;
-  const int lo = blockIdx.y;
+  // x pass: 1-D grid remapped so that the column tiles of one point run back
+  // to back on one XCD (blocks are dealt round-robin to the 8 XCDs), where
+  // the point's Green tile then stays in that XCD's L2.
+  int pblk = blockIdx.x, lo_blk = blockIdx.y;
+  if (MODE == PASS_CONV_X) {
+    const int b = blockIdx.x, xcd = b & 7, slot = b >> 3;
+    pblk = (slot / lp.nlo) * 8 + xcd;
+    lo_blk = slot % lp.nlo;
+    if (pblk * P >= lp.npts) return;
+  }
+  const int pt0 = pblk * P;
   const int np_here = min(P, lp.npts - pt0);
   const bool active = p < np_here;
   const int pt = pt0 + p;
   const V zero = cx<V>(0, 0);
+  const bool s1 = w < R2, s2 = w < R1;
+  const int j2 = w, k1 = w;
   for (int i = tid; i < M; i += blockDim.x) tws[i] = reinterpret_cast<const V*>(lp.tw)[i];
   if (MODE == PASS_CONV_X) {
-    // half spectrum stored x-innermost: [(ay*hz + az)*hx + ax][6]
+    // half spectrum stored x-innermost: [(ay*hz + az)*hx + ax][6]; the
+    // odd components (xy, xz, yz) flip sign under the mirror q -> m - q
     const V* __restrict__ G = reinterpret_cast<const V*>(lp.green);
     for (int i = tid; i < np_here * M * 6; i += blockDim.x) {
       const int comp = i % 6, qx = (i / 6) % M, pp = i / (6 * M);
@@ -441,12 +453,10 @@ __global__ void __launch_bounds__(512) k_reg2(LinePass lp) {
       gls[i] = g;
     }
   }
-
-  // ---- step 1 (forward or inverse by mode), input from global ---------------
-  V a[RMAX];
-  const bool s1 = w < R2;
-  const int j2 = w;
   {
+    const int lo = lo_blk;
+    // ---- step 1 input from global ---------------------------------------------
+    V a[RMAX];
     const V* __restrict__ src = reinterpret_cast<const V*>(lp.src) + (long long)lo * lp.lo_src;
     if (MODE == PASS_SCATTER_Z || MODE == PASS_CUBOID_Z) {
       const int c = q / lp.Lb, l = lo * lp.Lb + q % lp.Lb;
@@ -472,110 +482,108 @@ __global__ void __launch_bounds__(512) k_reg2(LinePass lp) {
         a[j1] = (s1 && active && e < lp.nin) ? base[e * lp.se] : zero;
       }
     }
-  }
-  __syncthreads();  // twiddles and Green tile staged
-  if (s1) {
-    reg2_step1<R1, R2, INV_PASS>(a, tws, j2);
-#pragma unroll
-    for (int k1 = 0; k1 < R1; ++k1) buf[(k1 * R2 + j2) * QQ + qq] = a[k1];
-  }
-  __syncthreads();
-
-  // ---- step 2 -----------------------------------------------------------------
-  const bool s2 = w < R1;
-  const int k1 = w;
-  if (s2) {
-#pragma unroll
-    for (int jj = 0; jj < R2; ++jj) a[jj] = buf[(k1 * R2 + jj) * QQ + qq];
-    Dft<R2, INV_PASS, V>::run(a);
-  }
-  if (MODE == PASS_CONV_X) {
-    // X[k1 + R1 k2] -> LDS natural order, 3x3 multiply, inverse transform
-    __syncthreads();
-    if (s2) {
-#pragma unroll
-      for (int k2 = 0; k2 < R2; ++k2) buf[(k1 + R1 * k2) * QQ + qq] = a[k2];
-    }
-    __syncthreads();
-    const int Lb = lp.Lb;
-    for (int it = tid; it < np_here * Lb * M; it += blockDim.x) {
-      const int li = it % Lb, kx = (it / Lb) % M, pp = it / (Lb * M);
-      const V* g = gls + (pp * M + kx) * 6;
-      V* row = buf + kx * QQ + pp * Q;
-      const V x0 = row[li], x1 = row[Lb + li], x2 = row[2 * Lb + li];
-      row[li] = cadd(cadd(cmul(g[0], x0), cmul(g[1], x1)), cmul(g[2], x2));
-      row[Lb + li] = cadd(cadd(cmul(g[1], x0), cmul(g[3], x1)), cmul(g[4], x2));
-      row[2 * Lb + li] = cadd(cadd(cmul(g[2], x0), cmul(g[4], x1)), cmul(g[5], x2));
-    }
-    __syncthreads();
-    // inverse step 1 from LDS natural order
+    __syncthreads();  // staged tables ready; previous tile's LDS reads done
     if (s1) {
-#pragma unroll
-      for (int j1 = 0; j1 < R1; ++j1) a[j1] = buf[(j1 * R2 + j2) * QQ + qq];
-    }
-    __syncthreads();
-    if (s1) {
-      reg2_step1<R1, R2, true>(a, tws, j2);
+      reg2_step1<R1, R2, INV_PASS>(a, tws, j2);
 #pragma unroll
       for (int kk = 0; kk < R1; ++kk) buf[(kk * R2 + j2) * QQ + qq] = a[kk];
     }
     __syncthreads();
+    // ---- step 2 ---------------------------------------------------------------
     if (s2) {
 #pragma unroll
       for (int jj = 0; jj < R2; ++jj) a[jj] = buf[(k1 * R2 + jj) * QQ + qq];
-      Dft<R2, true, V>::run(a);
+      Dft<R2, INV_PASS, V>::run(a);
     }
-  }
-
-  // ---- store X[k1 + R1 k2], k < nout --------------------------------------------
-  if (!s2 || !active) return;
-  if (MODE == PASS_GATHER_Z || MODE == PASS_CROP_Z) {
-    using T = typename RealOf<V>::T;
-    const T sc = (T)lp.scale;
-    const int c = q / lp.Lb, l = lo * lp.Lb + q % lp.Lb;
-    if (l >= lp.L) return;
-    V* __restrict__ out = reinterpret_cast<V*>(lp.AS);
-    long long row[R2];
+    if (MODE == PASS_CONV_X) {
+      // X[k1 + R1 k2] -> LDS natural order, 3x3 multiply, inverse transform
+      __syncthreads();
+      if (s2) {
 #pragma unroll
-    for (int k2 = 0; k2 < R2; ++k2) {
-      const int e = k1 + R1 * k2;
-      row[k2] = -1;
-      if (e < lp.nout) {
-        const long long cell = (long long)pt * lp.nz + e;
-        const long long m = (MODE == PASS_GATHER_Z) ? (long long)lp.inv[cell] : cell;
-        if (m >= 0) row[k2] = 3 * m + c;
+        for (int k2 = 0; k2 < R2; ++k2) buf[(k1 + R1 * k2) * QQ + qq] = a[k2];
+      }
+      __syncthreads();
+      const int Lb = lp.Lb;
+      for (int t = tid; t < np_here * Lb * M; t += blockDim.x) {
+        const int li = t % Lb, kx = (t / Lb) % M, pp = t / (Lb * M);
+        const V* g = gls + (pp * M + kx) * 6;
+        V* row = buf + kx * QQ + pp * Q;
+        const V x0 = row[li], x1 = row[Lb + li], x2 = row[2 * Lb + li];
+        row[li] = cadd(cadd(cmul(g[0], x0), cmul(g[1], x1)), cmul(g[2], x2));
+        row[Lb + li] = cadd(cadd(cmul(g[1], x0), cmul(g[3], x1)), cmul(g[4], x2));
+        row[2 * Lb + li] = cadd(cadd(cmul(g[2], x0), cmul(g[4], x1)), cmul(g[5], x2));
+      }
+      __syncthreads();
+      if (s1) {
+#pragma unroll
+        for (int j1 = 0; j1 < R1; ++j1) a[j1] = buf[(j1 * R2 + j2) * QQ + qq];
+      }
+      __syncthreads();
+      if (s1) {
+        reg2_step1<R1, R2, true>(a, tws, j2);
+#pragma unroll
+        for (int kk = 0; kk < R1; ++kk) buf[(kk * R2 + j2) * QQ + qq] = a[kk];
+      }
+      __syncthreads();
+      if (s2) {
+#pragma unroll
+        for (int jj = 0; jj < R2; ++jj) a[jj] = buf[(k1 * R2 + jj) * QQ + qq];
+        Dft<R2, true, V>::run(a);
       }
     }
-    if (MODE == PASS_GATHER_Z) {
-      const V* __restrict__ S = reinterpret_cast<const V*>(lp.S);
-      const V* __restrict__ dg = reinterpret_cast<const V*>(lp.diag);
-      const V* __restrict__ jp = reinterpret_cast<const V*>(lp.jpre);
-      V sv[R2], dv[R2], jv[R2];
+    // ---- store X[k1 + R1 k2], k < nout ---------------------------------------------
+    if (s2 && active) {
+      if (MODE == PASS_GATHER_Z || MODE == PASS_CROP_Z) {
+        using T = typename RealOf<V>::T;
+        const T sc = (T)lp.scale;
+        const int c = q / lp.Lb, l = lo * lp.Lb + q % lp.Lb;
+        if (l < lp.L) {
+          V* __restrict__ out = reinterpret_cast<V*>(lp.AS);
+          long long row[R2];
 #pragma unroll
-      for (int k2 = 0; k2 < R2; ++k2) {
-        if (row[k2] >= 0) {
-          sv[k2] = S[row[k2] * lp.L + l];
-          dv[k2] = dg[row[k2]];
-          jv[k2] = jp[row[k2]];
+          for (int k2 = 0; k2 < R2; ++k2) {
+            const int e = k1 + R1 * k2;
+            row[k2] = -1;
+            if (e < lp.nout) {
+              const long long cell = (long long)pt * lp.nz + e;
+              const long long m = (MODE == PASS_GATHER_Z) ? (long long)lp.inv[cell] : cell;
+              if (m >= 0) row[k2] = 3 * m + c;
+            }
+          }
+          if (MODE == PASS_GATHER_Z) {
+            const V* __restrict__ S = reinterpret_cast<const V*>(lp.S);
+            const V* __restrict__ dg = reinterpret_cast<const V*>(lp.diag);
+            const V* __restrict__ jp = reinterpret_cast<const V*>(lp.jpre);
+            V sv[R2], dv[R2], jv[R2];
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) {
+              if (row[k2] >= 0) {
+                sv[k2] = S[row[k2] * lp.L + l];
+                dv[k2] = dg[row[k2]];
+                jv[k2] = jp[row[k2]];
+              }
+            }
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2)
+              if (row[k2] >= 0)
+                out[row[k2] * lp.L + l] = cmul(jv[k2], cadd(cmul(dv[k2], sv[k2]), cscale(a[k2], sc)));
+          } else {
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2)
+              if (row[k2] >= 0) out[row[k2] * lp.L + l] = cscale(a[k2], sc);
+          }
+        }
+      } else {
+        V* __restrict__ dst = reinterpret_cast<V*>(lp.dst) + (long long)lo * lp.lo_dst;
+        const int ga = pt / lp.ptb, gb = pt % lp.ptb;
+        V* __restrict__ obase = dst + ga * lp.da + gb * lp.db + q;
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) {
+          const int e = k1 + R1 * k2;
+          if (e < lp.nout) obase[e * lp.de] = a[k2];
         }
       }
-#pragma unroll
-      for (int k2 = 0; k2 < R2; ++k2)
-        if (row[k2] >= 0) out[row[k2] * lp.L + l] = cmul(jv[k2], cadd(cmul(dv[k2], sv[k2]), cscale(a[k2], sc)));
-    } else {
-#pragma unroll
-      for (int k2 = 0; k2 < R2; ++k2)
-        if (row[k2] >= 0) out[row[k2] * lp.L + l] = cscale(a[k2], sc);
     }
-    return;
-  }
-  V* __restrict__ dst = reinterpret_cast<V*>(lp.dst) + (long long)lo * lp.lo_dst;
-  const int ga = pt / lp.ptb, gb = pt % lp.ptb;
-  V* __restrict__ obase = dst + ga * lp.da + gb * lp.db + q;
-#pragma unroll
-  for (int k2 = 0; k2 < R2; ++k2) {
-    const int e = k1 + R1 * k2;
-    if (e < lp.nout) obase[e * lp.de] = a[k2];
   }
 }
 
@@ -615,7 +623,10 @@ static void launch_reg2(LinePass lp, hipStream_t s) {
   lp.P = P;
   const int nthr = P * lp.Q * RMAX;
   const size_t lds = ((size_t)M * P * lp.Q + M + (conv ? (size_t)P * M * 6 : 0)) * sizeof(V);
-  dim3 grid((lp.npts + P - 1) / P, lp.nlo > 0 ? lp.nlo : 1);
+  const int nblk = (lp.npts + P - 1) / P;
+  const int nlo = lp.nlo > 0 ? lp.nlo : 1;
+  dim3 grid(nblk, nlo);
+  if (conv) grid = dim3(((nblk + 7) / 8) * 8 * nlo, 1);
   const int mode = (lp.mode == PASS_PLAIN && lp.inverse) ? PASS_PLAIN_INV : lp.mode;
   switch (mode) {
     case PASS_PLAIN: launch_reg2_mode<V, R1, R2, PASS_PLAIN>(lp, grid, nthr, lds, s); break;
