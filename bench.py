@@ -121,6 +121,8 @@ def main():
     import numpy as np
     import torch
     from bdda_amd import workloads
+    from bl_dda.scatterer import DiscreteDipoles
+    DiscreteDipoles.verbose_default = False
     wl = workloads.FULL_64
     log("building geometry")
     geometry = workloads.build_geometry(wl)

@@ -44,6 +44,17 @@ _SIGNATURES = {
     "bdda_observables": ([_vp, _vp, _i, _vp, _vp, _d, _vp, _vp], _i),
     "bdda_nearest_distance": ([_vp, _i64, _vp, _i64, _vp], _i),
     "bdda_inside_mask": ([_vp, _i, _i, _i, _d, _vp], _i),
+    "bdda_gemm_bf16": ([_i, _i, _i, _i, ctypes.c_float, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64,
+                        ctypes.c_float, _vp, _i64, _i64, _i64, _vp], _i),
+    "bdda_fno_inputs": ([_vp, _vp, _vp, _vp, _i, _i64, _vp], _i),
+    "bdda_cube_exchange": ([_vp, _vp, _vp, _vp, _i64, _i, _i, _i, _vp], _i),
+    "bdda_masked_l2": ([_vp, _vp, _vp, _vp, _i, _i64, _vp, _vp, _i, _vp], _i),
+    "bdda_gelu": ([_vp, _vp, _i64, _vp], _i),
+    "bdda_gelu_bwd": ([_vp, _vp, _vp, _i64, _vp], _i),
+    "bdda_bias_add": ([_vp, _vp, _i64, _i, _i64, _vp], _i),
+    "bdda_bias_grad": ([_vp, _vp, _i64, _i, _i64, _vp], _i),
+    "bdda_adam": ([_vp, _vp, _vp, _vp, _i64, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _i,
+                   _vp], _i),
 }
 
 _lib = None

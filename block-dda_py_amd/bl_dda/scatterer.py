@@ -75,7 +75,13 @@ class IncidentField:
 
 
 class DiscreteDipoles(Target, IncidentField):
-    """DDA system on the GPU (scatterer.py:88-228)."""
+    """DDA system on the GPU (scatterer.py:88-228).
+
+    ``DiscreteDipoles.verbose_default`` (True, like the reference's
+    unconditional prints) sets the initial ``verbose`` of new instances.
+    """
+
+    verbose_default = True
 
     def __init__(self, target, incidentfield):
         Target.__init__(self, target.shape_name, target.lattice_n, target.lattice_lf, target.lattice_grid_points,
@@ -107,7 +113,7 @@ class DiscreteDipoles(Target, IncidentField):
         self.refine_every = 20
         self.X0 = None
         self.pad_to = None
-        self.verbose = True
+        self.verbose = DiscreteDipoles.verbose_default
         self.iter_fin = None
         self.err_fin = None
         self.true_residual = None

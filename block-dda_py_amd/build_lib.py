@@ -26,7 +26,7 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", os.path.join(ROOT, "include"),
 ]
-SOURCES = ["kernels_fft.hip", "kernels_block.hip", "plan.hip"]
+SOURCES = ["kernels_fft.hip", "kernels_block.hip", "kernels_fno.hip", "plan.hip"]
 
 
 def _compile(src):

@@ -120,6 +120,32 @@ void launch_convert(Vd* dst, const Vs* src, long long n, double scale, hipStream
 void launch_add_c64_to_c128(double2* X, const float2* D, long long n, double scale, hipStream_t s);
 void launch_fill_inv(int* inv, long long ncell, const long long* addr, long long N, hipStream_t s);
 
+// learned warm start (float32 storage, bf16 MFMA)
+struct GemmArgs {
+  int batch, M, N, K;
+  float alpha, beta;
+  const float* A;
+  long long sAb, sAi, sAk;
+  const float* B;
+  long long sBb, sBk, sBj;
+  float* C;
+  long long sCb, sCi, sCj;
+};
+void launch_gemm_bf16(const GemmArgs& g, hipStream_t s);
+void launch_gelu(const float* x, float* y, long long n, hipStream_t s);
+void launch_gelu_bwd(const float* x, const float* dy, float* dx, long long n, hipStream_t s);
+void launch_bias_add(float* y, const float* b, long long B, int C, long long S, hipStream_t s);
+void launch_bias_grad(const float* dy, float* db, long long B, int C, long long S, hipStream_t s);
+void launch_adam(float* p, const float* g, float* m, float* v, long long n, float lr, float b1, float b2, float eps,
+                 float c1, float c2, hipStream_t s);
+
+void launch_fno_inputs(float* out, const unsigned char* occ, const int* sample_inst, const float* desc, int G,
+                       long long nsamp, hipStream_t s);
+void launch_cube_exchange(float* cube, double2* X, const long long* cells, const double2* alpha, long long N, int L,
+                          int G, int to_cube, hipStream_t s);
+void launch_masked_l2(const float* pred, const float* target, const unsigned char* occ, const int* sample_inst,
+                      int C, long long S, float* loss, float* grad, int nsamp, hipStream_t s);
+
 // physics
 void launch_rhs(double2* B, const double* pos, const double* u, const double2* e0, double k,
                 long long N, int L, hipStream_t s);

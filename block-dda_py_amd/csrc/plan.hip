@@ -1105,4 +1105,100 @@ int bdda_inside_mask(const double* dist, int nx, int ny, int nz, double thresh, 
   return 0;
 }
 
+// ----------------------------------------------------- learned warm start
+int bdda_gemm_bf16(int batch, int M, int N, int K, float alpha, const float* A, int64_t sAb, int64_t sAi,
+                   int64_t sAk, const float* B, int64_t sBb, int64_t sBk, int64_t sBj, float beta, float* C,
+                   int64_t sCb, int64_t sCi, int64_t sCj, void* stream) {
+  g_err.clear();
+  if (batch < 0 || M < 0 || N < 0 || K < 0 || !A || !B || !C) {
+    set_error("bdda_gemm_bf16: bad arguments");
+    return BDDA_E_ARG;
+  }
+  if (batch == 0 || M == 0 || N == 0) return 0;
+  GemmArgs g{0, M, N, K, alpha, beta, A, sAb, sAi, sAk, B, sBb, sBk, sBj, C, sCb, sCi, sCj};
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  for (int b0 = 0; b0 < batch; b0 += 65535) {
+    g.batch = std::min(65535, batch - b0);
+    g.A = A + (long long)b0 * sAb;
+    g.B = B + (long long)b0 * sBb;
+    g.C = C + (long long)b0 * sCb;
+    launch_gemm_bf16(g, s);
+    BDDA_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
+int bdda_fno_inputs(float* out, const uint8_t* occ, const int* sample_inst, const float* desc, int G, int64_t nsamp,
+                    void* stream) {
+  g_err.clear();
+  if (!out || !occ || !sample_inst || !desc || G < 1 || nsamp < 0) {
+    set_error("bdda_fno_inputs: bad arguments");
+    return BDDA_E_ARG;
+  }
+  launch_fno_inputs(out, occ, sample_inst, desc, G, nsamp, reinterpret_cast<hipStream_t>(stream));
+  BDDA_HIP(hipGetLastError());
+  return 0;
+}
+
+int bdda_cube_exchange(float* cube, void* X, const int64_t* cells, const void* alpha, int64_t N, int L, int G,
+                       int to_cube, void* stream) {
+  g_err.clear();
+  if (!cube || !X || !cells || !alpha || N < 0 || L < 1 || G < 1) {
+    set_error("bdda_cube_exchange: bad arguments");
+    return BDDA_E_ARG;
+  }
+  launch_cube_exchange(cube, (double2*)X, (const long long*)cells, (const double2*)alpha, N, L, G, to_cube,
+                       reinterpret_cast<hipStream_t>(stream));
+  BDDA_HIP(hipGetLastError());
+  return 0;
+}
+
+int bdda_masked_l2(const float* pred, const float* target, const uint8_t* occ, const int* sample_inst, int C,
+                   int64_t S, float* loss, float* grad, int nsamp, void* stream) {
+  g_err.clear();
+  if (!pred || !target || !occ || !sample_inst || !loss || nsamp < 1) {
+    set_error("bdda_masked_l2: bad arguments");
+    return BDDA_E_ARG;
+  }
+  launch_masked_l2(pred, target, occ, sample_inst, C, S, loss, grad, nsamp, reinterpret_cast<hipStream_t>(stream));
+  BDDA_HIP(hipGetLastError());
+  return 0;
+}
+
+int bdda_gelu(const float* x, float* y, int64_t n, void* stream) {
+  launch_gelu(x, y, n, reinterpret_cast<hipStream_t>(stream));
+  BDDA_HIP(hipGetLastError());
+  return 0;
+}
+
+int bdda_gelu_bwd(const float* x, const float* dy, float* dx, int64_t n, void* stream) {
+  launch_gelu_bwd(x, dy, dx, n, reinterpret_cast<hipStream_t>(stream));
+  BDDA_HIP(hipGetLastError());
+  return 0;
+}
+
+int bdda_bias_add(float* y, const float* b, int64_t B, int C, int64_t S, void* stream) {
+  launch_bias_add(y, b, B, C, S, reinterpret_cast<hipStream_t>(stream));
+  BDDA_HIP(hipGetLastError());
+  return 0;
+}
+
+int bdda_bias_grad(const float* dy, float* db, int64_t B, int C, int64_t S, void* stream) {
+  launch_bias_grad(dy, db, B, C, S, reinterpret_cast<hipStream_t>(stream));
+  BDDA_HIP(hipGetLastError());
+  return 0;
+}
+
+int bdda_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2, float eps,
+              int step, void* stream) {
+  if (step < 1) {
+    set_error("bdda_adam: step counts from 1");
+    return BDDA_E_ARG;
+  }
+  const float c1 = 1.f - std::pow(b1, (float)step), c2 = 1.f - std::pow(b2, (float)step);
+  launch_adam(p, g, m, v, n, lr, b1, b2, eps, c1, c2, reinterpret_cast<hipStream_t>(stream));
+  BDDA_HIP(hipGetLastError());
+  return 0;
+}
+
 }  // extern "C"

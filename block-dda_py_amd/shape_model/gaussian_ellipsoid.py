@@ -46,8 +46,15 @@ class gaussian_ellipsoid_shape_model:  # noqa: N801  (reference class name)
         self.name = ("GRE_shape__r_v_base={:.2f}um__ab_ratio={:.1f}"
                      "__bc_ratio={:.1f}__beta={:.2f}".format(self.r_v_base, self.ab_ratio, self.bc_ratio,
                                                             self.beta))
+        cov = self.beta ** 2 * self.correlation_matrix()
+        draw = rng.multivariate_normal(np.zeros(self.N_THETA * self.N_PHI), cov, size=1, method='eigh')
+        return self.surface_from_sample(draw.squeeze().reshape(self.N_THETA, self.N_PHI))
+
+    def correlation_matrix(self):
+        """exp(-d^2 / (2 lc^2)) on the coarse grid; independent of r_v and beta
+        (all distances scale with c, as does lc = 0.3 c)."""
         a, b, c = self._axes()
-        h0, lc = c ** 2 / a, c * 0.3
+        lc = c * 0.3
         th = np.linspace(0, np.pi, self.N_THETA)
         ph = np.linspace(0, 2 * np.pi, self.N_PHI)
         T, F = np.meshgrid(th, ph, indexing='ij')
@@ -56,9 +63,16 @@ class gaussian_ellipsoid_shape_model:  # noqa: N801  (reference class name)
         d2 = (coarse[0][:, None] - coarse[0][None, :]) ** 2
         d2 = d2 + (coarse[1][:, None] - coarse[1][None, :]) ** 2
         d2 = d2 + (coarse[2][:, None] - coarse[2][None, :]) ** 2
-        cov = self.beta ** 2 * np.exp(-0.5 * d2 / lc ** 2)
-        draw = rng.multivariate_normal(np.zeros(self.N_THETA * self.N_PHI), cov, size=1, method='eigh')
-        s = draw.squeeze().reshape(self.N_THETA, self.N_PHI)
+        return np.exp(-0.5 * d2 / lc ** 2)
+
+    def surface_from_sample(self, s):
+        """Surface points and meshes for a given coarse log-radius field s
+        (N_THETA x N_PHI): spline upsampling and normal displacement
+        (gaussian_ellipsoid.py:122-147)."""
+        a, b, c = self._axes()
+        h0 = c ** 2 / a
+        th = np.linspace(0, np.pi, self.N_THETA)
+        ph = np.linspace(0, 2 * np.pi, self.N_PHI)
         spline = scipy.interpolate.RectBivariateSpline(th, ph, s)
         up = int(4 * np.cbrt(self.bc_ratio * self.ab_ratio))
         th2 = np.linspace(0, np.pi, up * self.N_THETA)

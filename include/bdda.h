@@ -157,6 +157,44 @@ int bdda_nearest_distance(const double* grid_host, int64_t n, const double* surf
  * (shape_model/gaussian_ellipsoid.py:195-237). */
 int bdda_inside_mask(const double* dist_host, int nx, int ny, int nz, double thresh, uint8_t* inside_host);
 
+/* ---- Learned warm start (no counterpart in the reference: the reference
+ * has no learned component; these build the 3-D Fourier neural operator that
+ * predicts an initial guess for bdda_solve's X0_dev).  float32 device
+ * buffers. */
+
+/* C = alpha * A B + beta * C for `batch` independent products with
+ * arbitrary element strides: A[b*sAb + i*sAi + k*sAk] (M x K),
+ * B[b*sBb + k*sBk + j*sBj] (K x N), C[b*sCb + i*sCi + j*sCj] (M x N).
+ * Inputs are rounded to bfloat16 and multiplied on the MFMA units with
+ * float32 accumulation. */
+int bdda_gemm_bf16(int batch, int M, int N, int K, float alpha, const float* A, int64_t sAb, int64_t sAi,
+                   int64_t sAk, const float* B, int64_t sBb, int64_t sBk, int64_t sBj, float beta, float* C,
+                   int64_t sCb, int64_t sCi, int64_t sCj, void* stream);
+/* y = GELU(x) = 0.5 x (1 + erf(x / sqrt 2)); dx = dy * GELU'(x). */
+int bdda_gelu(const float* x, float* y, int64_t n, void* stream);
+int bdda_gelu_bwd(const float* x, const float* dy, float* dx, int64_t n, void* stream);
+/* (B, C, S) tensors: y[b][c][s] += bias[c];  db[c] = sum_{b,s} dy[b][c][s]. */
+int bdda_bias_add(float* y, const float* bias, int64_t B, int C, int64_t S, void* stream);
+int bdda_bias_grad(const float* dy, float* db, int64_t B, int C, int64_t S, void* stream);
+/* Network input channels for nsamp samples on a G^3 cube (out: nsamp x 14
+ * x G^3).  occ: per-instance occupancy cubes (bytes); sample_inst: instance
+ * of each sample; desc: 20 floats per sample (chi re/im x3, u (3), e0 re/im
+ * x3, k*lattice_lf, lattice origin / lattice_lf (3), unused). */
+int bdda_fno_inputs(float* out, const uint8_t* occ, const int* sample_inst, const float* desc, int G, int64_t nsamp,
+                    void* stream);
+/* Between a (3N, L) complex128 block X and an (L, 6, G^3) float32 cube of
+ * X / alpha (re, im per axis) at the given occupied cells: to_cube = 1
+ * writes the cube (other cells untouched), 0 writes X = alpha * cube. */
+int bdda_cube_exchange(float* cube, void* X, const int64_t* cells, const void* alpha, int64_t N, int L, int G,
+                       int to_cube, void* stream);
+/* Masked relative L2 loss per sample and (if grad != NULL) its gradient,
+ * averaged over samples; pred/target (nsamp, C, S), occ (instances, S). */
+int bdda_masked_l2(const float* pred, const float* target, const uint8_t* occ, const int* sample_inst, int C,
+                   int64_t S, float* loss, float* grad, int nsamp, void* stream);
+/* One Adam step (step counts from 1) on n parameters. */
+int bdda_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2, float eps,
+              int step, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

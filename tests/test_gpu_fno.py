@@ -1,0 +1,132 @@
+"""GPU checks of the learned warm start: the bf16 MFMA GEMM against numpy
+with the same bf16 rounding (pins the MFMA lane layout), the FNO forward
+against the float64 CPU oracle, and the hand-written backward against finite
+differences of the oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():
+    pytest.skip("no GPU", allow_module_level=True)
+
+from oracle import fno as ofno  # noqa: E402
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def bf16(a):
+    """Round float32 values to bfloat16 (nearest even), returned as float64."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("shape", [(1, 16, 16, 16), (3, 37, 70, 29), (5, 64, 64, 64), (2, 130, 3, 200)])
+def test_gemm_bf16_exact_layout(shape):
+    say(f"gemm {shape}")
+    from bdda_amd.fno import Gemm
+    b, M, N, K = shape
+    rng = np.random.default_rng(0)
+    A = rng.standard_normal((b, K, M)).astype(np.float32)   # stored transposed: A[b][k][i]
+    B = rng.standard_normal((b, N, K)).astype(np.float32)   # stored transposed: B[b][j][k]
+    C0 = rng.standard_normal((b, M, N)).astype(np.float32)
+    ta, tb, tc = (torch.from_numpy(x.copy()).cuda() for x in (A, B, C0))
+    Gemm()(b, M, N, K, (ta, 0, (K * M, 1, M)), (tb, 0, (N * K, 1, K)), (tc, 0, (M * N, N, 1)), alpha=0.5, beta=-2.0)
+    got = tc.cpu().numpy()
+    ref = 0.5 * np.einsum('bki,bjk->bij', bf16(A), bf16(B)) - 2.0 * C0.astype(np.float64)
+    scale = np.sqrt(K) * np.abs(bf16(A)).max() * np.abs(bf16(B)).max()
+    assert np.max(np.abs(got - ref)) < 1e-5 * scale
+
+
+def _tiny(G=8, C=4, K=4, nl=2, hid=8, seed=1):
+    from bdda_amd.fno import FNO
+    m = FNO(G=G, width=C, modes=K, layers=nl, hidden=hid, seed=seed)
+    # larger spectral weights than the default init so the spectral path matters
+    rng = np.random.default_rng(seed + 7)
+    for l in range(nl):
+        for nm in ("Rr", "Ri"):
+            m.params[f"l{l}.{nm}"] = torch.tensor(rng.standard_normal((K ** 3, C, C)) * 0.2, dtype=torch.float32,
+                                                  device="cuda")
+    p64 = {k: v.cpu().numpy().astype(np.float64) for k, v in m.params.items()}
+    return m, p64
+
+
+def test_fno_forward_matches_oracle():
+    say("fno forward")
+    G, K, nl = 8, 4, 2
+    m, p64 = _tiny(G=G, K=K, nl=nl)
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal((3, 14, G ** 3)).astype(np.float32)
+    y = m.forward(torch.from_numpy(x).cuda()).cpu().numpy()
+    ref = ofno.forward(p64, x.astype(np.float64), G, K, nl)
+    assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 2e-2
+
+
+def test_fno_gradient_matches_oracle_finite_differences():
+    say("fno gradient")
+    G, K, nl = 8, 4, 2
+    m, p64 = _tiny(G=G, K=K, nl=nl)
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((2, 14, G ** 3)).astype(np.float32)
+    w = rng.standard_normal((2, 6, G ** 3)).astype(np.float32)
+    m.forward(torch.from_numpy(x).cuda(), keep=True)
+    grads = {k: v.cpu().numpy().astype(np.float64) for k, v in m.backward(torch.from_numpy(w).cuda()).items()}
+    x64, w64 = x.astype(np.float64), w.astype(np.float64)
+    # bf16 GEMM inputs give a few percent noise per directional derivative,
+    # so each parameter block is checked over 8 random directions: the
+    # analytic values must track the float64 finite differences (a sign,
+    # conjugation or transposition mistake gives O(1) errors)
+    report = []
+    for name in ["lift.W", "lift.b", "l0.Rr", "l0.Ri", "l0.W", "l1.Rr", "l1.Ri", "l1.W", "l1.b", "p1.W", "p2.W",
+                 "p2.b"]:
+        an, fd = [], []
+        for _ in range(8):
+            d = rng.standard_normal(p64[name].shape)
+            eps = 1e-4
+            pp = dict(p64)
+            pp[name] = p64[name] + eps * d
+            lp = ofno.loss(pp, x64, w64, G, K, nl)
+            pp[name] = p64[name] - eps * d
+            lm = ofno.loss(pp, x64, w64, G, K, nl)
+            fd.append((lp - lm) / (2 * eps))
+            an.append(float(np.sum(grads[name] * d)))
+        an, fd = np.array(an), np.array(fd)
+        err = np.linalg.norm(an - fd) / np.linalg.norm(fd)
+        corr = np.corrcoef(an, fd)[0, 1]
+        report.append((name, round(float(err), 4), round(float(corr), 5)))
+    print(report)
+    for name, err, corr in report:
+        assert err < 0.12 and corr > 0.98, report
+
+
+def test_fno_training_reduces_loss():
+    say("fno training")
+    import ctypes
+    from bdda_amd import _lib
+    from bdda_amd.fno import _p, _stream
+    G, K = 8, 4
+    m, _ = _tiny(G=G, K=K, nl=2)
+    rng = np.random.default_rng(4)
+    x = torch.tensor(rng.standard_normal((4, 14, G ** 3)), dtype=torch.float32, device="cuda")
+    # a smooth target the operator can represent
+    t = torch.tensor(np.tile(np.sin(np.arange(G ** 3) * 0.01), (4, 6, 1)), dtype=torch.float32, device="cuda")
+    occ = torch.ones(G ** 3, dtype=torch.uint8, device="cuda")
+    inst = torch.zeros(4, dtype=torch.int32, device="cuda")
+    lib = _lib.load()
+    losses = []
+    for step in range(100):
+        y = m.forward(x, keep=True)
+        loss = torch.empty(4, dtype=torch.float32, device="cuda")
+        dy = torch.empty_like(y)
+        _lib.check(lib.bdda_masked_l2(_p(y), _p(t), ctypes.c_void_p(occ.data_ptr()),
+                                      ctypes.c_void_p(inst.data_ptr()), 6, G ** 3, _p(loss), _p(dy), 4, _stream()))
+        losses.append(float(loss.mean()))
+        m.adam_step(m.backward(dy), lr=3e-3)
+    assert losses[-1] < 0.75 * losses[0] and losses[-1] < min(losses[:10]), losses[::10]

@@ -240,3 +240,26 @@ def test_mie_rayleigh_limit_and_optical_theorem():
     Qs2, Qa2, Qe2, Sf2, Sb2 = mie.mie_q_and_s(wl, mm, r, 1.5 + 0.01j)
     k2 = 2 * np.pi * mm / wl
     np.testing.assert_allclose(Qe2 * np.pi * r ** 2, 4 * np.pi / k2 * float(np.imag(Sf2)), rtol=1e-6)
+
+
+# ------------------------------------------------------- learned warm start
+def test_fno_oracle_spectral_equals_direct_dft():
+    from oracle import fno
+    G, K, C = 4, 2, 2
+    rng = np.random.default_rng(0)
+    v = rng.standard_normal((1, C, G ** 3))
+    Rr = rng.standard_normal((K ** 3, C, C))
+    Ri = rng.standard_normal((K ** 3, C, C))
+    ks = fno.mode_indices(G, K)
+    n = np.arange(G)
+    idx = np.stack(np.meshgrid(n, n, n, indexing="ij"), -1).reshape(-1, 3)
+    out = np.zeros((1, C, G ** 3))
+    R = (Rr + 1j * Ri).reshape(K, K, K, C, C)
+    for a, kx in enumerate(ks):
+        for b, ky in enumerate(ks):
+            for c, kz in enumerate(ks):
+                ph = np.exp(-2j * np.pi * (idx @ np.array([kx, ky, kz])) / G)
+                vh = v[0] @ ph                       # (C,)
+                yh = vh @ R[a, b, c]                 # (C,)
+                out[0] += (yh[:, None] * np.conj(ph)[None, :]).real / G ** 3
+    np.testing.assert_allclose(fno.spectral(v, Rr, Ri, G, K), out, rtol=1e-12, atol=1e-12)
