@@ -622,6 +622,10 @@ static void launch_reg2(LinePass lp, hipStream_t s) {
   if (P > lp.npts) P = lp.npts;
   lp.P = P;
   const int nthr = P * lp.Q * RMAX;
+  if (nthr > 512) {
+    set_error("register FFT launch exceeds 512 threads");
+    return;
+  }
   const size_t lds = ((size_t)M * P * lp.Q + M + (conv ? (size_t)P * M * 6 : 0)) * sizeof(V);
   const int nblk = (lp.npts + P - 1) / P;
   const int nlo = lp.nlo > 0 ? lp.nlo : 1;
@@ -640,22 +644,30 @@ static void launch_reg2(LinePass lp, hipStream_t s) {
 }
 
 // Lengths with a two-step register kernel (m = R1 * R2); others use the
-// runtime-radix Stockham kernel.
+// runtime-radix Stockham kernel.  A register kernel needs Q * max(R1, R2)
+// threads for one point, which must stay within its 512-thread bound;
+// otherwise the length also goes to the Stockham kernel.
 template <typename V>
 static bool try_reg2(const LinePass& lp, hipStream_t s) {
-  if (lp.Q * 16 > 512 && lp.m >= 128) return false;
-  switch (lp.m) {
-    case 16: launch_reg2<V, 4, 4>(lp, s); return true;
-    case 24: launch_reg2<V, 8, 3>(lp, s); return true;
-    case 32: launch_reg2<V, 8, 4>(lp, s); return true;
-    case 40: launch_reg2<V, 8, 5>(lp, s); return true;
-    case 48: launch_reg2<V, 16, 3>(lp, s); return true;
-    case 64: launch_reg2<V, 8, 8>(lp, s); return true;
-    case 80: launch_reg2<V, 16, 5>(lp, s); return true;
-    case 128: launch_reg2<V, 16, 8>(lp, s); return true;
-    case 256: launch_reg2<V, 16, 16>(lp, s); return true;
-    default: return false;
+  static const int table[][3] = {{16, 4, 4}, {24, 8, 3},  {32, 8, 4},   {40, 8, 5},  {48, 16, 3},
+                                 {64, 8, 8}, {80, 16, 5}, {128, 16, 8}, {256, 16, 16}};
+  for (const auto& t : table) {
+    if (t[0] != lp.m) continue;
+    const int rmax = t[1] > t[2] ? t[1] : t[2];
+    if (lp.Q * rmax > 512) return false;
+    switch (lp.m) {
+      case 16: launch_reg2<V, 4, 4>(lp, s); return true;
+      case 24: launch_reg2<V, 8, 3>(lp, s); return true;
+      case 32: launch_reg2<V, 8, 4>(lp, s); return true;
+      case 40: launch_reg2<V, 8, 5>(lp, s); return true;
+      case 48: launch_reg2<V, 16, 3>(lp, s); return true;
+      case 64: launch_reg2<V, 8, 8>(lp, s); return true;
+      case 80: launch_reg2<V, 16, 5>(lp, s); return true;
+      case 128: launch_reg2<V, 16, 8>(lp, s); return true;
+      default: launch_reg2<V, 16, 16>(lp, s); return true;
+    }
   }
+  return false;
 }
 
 template <typename V>
@@ -676,6 +688,10 @@ void launch_line_pass(const LinePass& lp0, hipStream_t s) {
   if (P > lp.npts) P = lp.npts;
   lp.P = P;
   const int nthr = P * lp.Q * TPC;
+  if (nthr > 512) {
+    set_error("line FFT tile too large (launch exceeds 512 threads)");
+    return;
+  }
   const size_t lds = ((size_t)lp.m * P * lp.Q + lp.m) * sizeof(V);
   dim3 grid((lp.npts + P - 1) / P, lp.nlo > 0 ? lp.nlo : 1);
   switch (lp.mode) {

@@ -32,7 +32,8 @@ using la::zc;
       set_error(std::string("kernel launch: ") + hipGetErrorString(e_));                    \
       return BDDA_E_HIP;                                                                    \
     }                                                                                       \
-    if (!g_err.empty() && g_err.rfind("line FFT", 0) == 0) return BDDA_E_ARG;               \
+    if (!g_err.empty() && (g_err.rfind("line FFT", 0) == 0 || g_err.rfind("register FFT", 0) == 0)) \
+      return BDDA_E_ARG;                                                                    \
   } while (0)
 
 // Device buffer owned by a plan.

@@ -75,9 +75,16 @@ def main():
     ap.add_argument("--grid", type=int, default=32)
     ap.add_argument("--eval", type=int, default=4)
     ap.add_argument("--out", default=os.path.join(HERE, "bdda_amd", "data", "fno_gre32.npz"))
+    ap.add_argument("--eval-only", action="store_true", help="only measure the gain of the weights at --out")
     a = ap.parse_args()
     from bdda_amd.fno import FNO
     from bdda_amd.fno_train import generate, train
+    from bl_dda.scatterer import DiscreteDipoles
+    DiscreteDipoles.verbose_default = False
+    if a.eval_only:
+        model = FNO.load(a.out)
+        print(json.dumps(dict(weights=a.out, gain=gain(model, n_eval=a.eval, G=a.grid))), flush=True)
+        return
     t0 = time.time()
     data = generate(a.instances, G=a.grid, L=a.orientations, log=log)
     t_gen = time.time() - t0

@@ -130,3 +130,41 @@ def test_fno_training_reduces_loss():
         losses.append(float(loss.mean()))
         m.adam_step(m.backward(dy), lr=3e-3)
     assert losses[-1] < 0.75 * losses[0] and losses[-1] < min(losses[:10]), losses[::10]
+
+
+def test_trained_warm_start_reduces_iterations():
+    """The committed network (trained on 2000 solver-generated particles)
+    against a zero start on held-out isotropic particles: mixed-precision
+    block-COCG-rq to 1e-5 at L = 64 orientations.  The warm start must save
+    iterations on every particle and reach the same true residual, and a
+    guess worse than zero must be discarded (zero-start iteration count)."""
+    say("trained warm start")
+    import os
+    import numpy as np
+    from bdda_amd.fno import FNO
+    from bdda_amd.fno_train import ShapeBank, random_dipoles, warm_start
+    from bl_dda.scatterer import DiscreteDipoles, IncidentField, Target
+    from conftest import PKG
+    model = FNO.load(os.path.join(PKG, "bdda_amd", "data", "fno_gre32.npz"))
+    rng = np.random.default_rng(12345)  # training draws use seed 0
+    bank = ShapeBank(12346)
+    saved = []
+    for _ in range(2):
+        d0 = random_dipoles(rng, bank, 32, 64)
+        tg = Target("eval", d0.lattice_n, d0.lattice_lf, d0.lattice_grid_points,
+                    d0.lattice_grid_points_is_in_target, np.array([d0.m_p_xyz[0]] * 3))
+        dd = DiscreteDipoles(tg, IncidentField(d0.wl_0, d0.m_m, d0.euler_angles))
+        dd.verbose = False
+        dd.set_interaction_matrix()
+        dd.solver, dd.precision, dd.tol, dd.itermax = "cocg_rq", "mixed", 1e-5, 300
+        dd.solve_matrix_equation()
+        it0 = dd.iter_fin
+        X0 = warm_start(model, dd)
+        dd.X0 = X0
+        dd.solve_matrix_equation()
+        assert dd.warm_start_used and dd.converge and dd.true_residual < 1e-5
+        saved.append(it0 - dd.iter_fin)
+        dd.X0 = X0 * 1000.0
+        dd.solve_matrix_equation()
+        assert not dd.warm_start_used and dd.iter_fin == it0
+    assert min(saved) >= 2 and sum(saved) >= 6, saved

@@ -78,6 +78,23 @@ def test_register_and_stockham_kernels_agree(n):
         assert rel(out[path][1], ref) < 2e-6
 
 
+def test_wide_column_tile_on_register_lengths():
+    """L = 64 on a small grid selects 16-column tiles (Q = 48); lengths 48
+    (16 x 3) and 80 (16 x 5) would need 768 threads in the register kernel
+    and must take the Stockham kernel instead."""
+    say("wide tiles")
+    from mvp_fft import mvp_fft as pm
+    n, L = (24, 3, 5), 64
+    lf, k = 0.03, 9.0
+    rng = np.random.default_rng(5)
+    ncell = int(np.prod(n))
+    P = rng.standard_normal((3 * ncell, L)) + 1j * rng.standard_normal((3 * ncell, L))
+    ref = omvp.fft_mvp(n, 3, omvp.fft_init(n, 3, lf, k), P)
+    At = pm.fft_init(n, 3, lf, k)
+    assert rel(pm.fft_mvp(n, 3, At, P), ref) < 1e-12
+    assert rel(pm.fft_mvp(n, 3, At, P, precision="mixed"), ref) < 2e-6
+
+
 def test_green_spectrum_and_tensor_match_oracle():
     say("green spectrum")
     from mvp_fft import mvp_fft as pm
