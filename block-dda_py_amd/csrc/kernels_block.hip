@@ -121,56 +121,55 @@ template void launch_gram<float2>(const float2*, const float2*, long long, int, 
                                   hipStream_t);
 
 // ------------------------------------------------------------ block update
-// Y = a X + (W1 + w2c W2) C + e E  over 16-row slabs, output columns tiled by 64.
+// Y = a X + (W1 + w2c W2) C + e E  over 16-row slabs, output columns tiled by
+// 64.  The L x L coefficients, the combination and the accumulation stay in
+// float64 for both vector types (gfx950 issues float64 FMAs at the float32
+// rate), so a complex64 update is rounded once, at the store: with
+// ill-conditioned coefficients a float accumulation would amplify rounding.
 template <typename V>
 __global__ void __launch_bounds__(256) k_update(V* Y, const V* X, double2 a, const V* W1, const V* W2,
                                                 double2 w2c, const double2* __restrict__ C, const V* E,
                                                 double2 e, long long rows, int L) {
   using T = typename RealOf<V>::T;
   extern __shared__ __align__(16) unsigned char sm[];
-  V* Cs = reinterpret_cast<V*>(sm);      // [L][64]
-  V* Ws = Cs + (size_t)L * 64;           // [16][L]
+  double2* Cs = reinterpret_cast<double2*>(sm);  // [L][64]
+  double2* Ws = Cs + (size_t)L * 64;              // [16][L]
   const int lt = blockIdx.y;
   const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
   for (int idx = threadIdx.x; idx < L * 64; idx += 256) {
     const int j = idx / 64, c = idx % 64;
     const int l = lt * 64 + c;
-    Cs[idx] = from_d2<V>(l < L ? C[j * L + l] : make_double2(0.0, 0.0));
+    Cs[idx] = l < L ? C[j * L + l] : make_double2(0.0, 0.0);
   }
-  const V wc = from_d2<V>(w2c);
   for (long long r0 = blockIdx.x * 16LL; r0 < rows; r0 += 16LL * gridDim.x) {
     __syncthreads();
     for (int idx = threadIdx.x; idx < 16 * L; idx += 256) {
       const int rr = idx / L, j = idx % L;
       const long long row = r0 + rr;
-      V w;
-      w.x = 0;
-      w.y = 0;
+      double2 w = make_double2(0.0, 0.0);
       if (row < rows) {
-        w = W1[row * L + j];
+        const V w1 = W1[row * L + j];
+        w = make_double2((double)w1.x, (double)w1.y);
         if (W2) {
           const V w2 = W2[row * L + j];
-          w.x += wc.x * w2.x - wc.y * w2.y;
-          w.y += wc.x * w2.y + wc.y * w2.x;
+          w.x += w2c.x * (double)w2.x - w2c.y * (double)w2.y;
+          w.y += w2c.x * (double)w2.y + w2c.y * (double)w2.x;
         }
       }
       Ws[idx] = w;
     }
     __syncthreads();
     const long long row = r0 + ty;
-    V acc[4];
+    double2 acc[4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      acc[t].x = 0;
-      acc[t].y = 0;
-    }
-    const V* wr = Ws + ty * L;
+    for (int t = 0; t < 4; ++t) acc[t] = make_double2(0.0, 0.0);
+    const double2* wr = Ws + ty * L;
     for (int j = 0; j < L; ++j) {
-      const V w = wr[j];
-      const V* cr = Cs + j * 64 + tx * 4;
+      const double2 w = wr[j];
+      const double2* cr = Cs + j * 64 + tx * 4;
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        const V c = cr[t];
+        const double2 c = cr[t];
         acc[t].x += w.x * c.x - w.y * c.y;
         acc[t].y += w.x * c.y + w.y * c.x;
       }
@@ -181,18 +180,21 @@ __global__ void __launch_bounds__(256) k_update(V* Y, const V* X, double2 a, con
         const int l = lt * 64 + tx * 4 + t;
         if (l >= L) continue;
         const long long o = row * L + l;
-        V y = acc[t];
+        double2 y = acc[t];
         if (X) {
           const V x = X[o];
-          y.x += (T)a.x * x.x - (T)a.y * x.y;
-          y.y += (T)a.x * x.y + (T)a.y * x.x;
+          y.x += a.x * (double)x.x - a.y * (double)x.y;
+          y.y += a.x * (double)x.y + a.y * (double)x.x;
         }
         if (E) {
           const V ev = E[o];
-          y.x += (T)e.x * ev.x - (T)e.y * ev.y;
-          y.y += (T)e.x * ev.y + (T)e.y * ev.x;
+          y.x += e.x * (double)ev.x - e.y * (double)ev.y;
+          y.y += e.x * (double)ev.y + e.y * (double)ev.x;
         }
-        Y[o] = y;
+        V out;
+        out.x = (T)y.x;
+        out.y = (T)y.y;
+        Y[o] = out;
       }
     }
   }
@@ -201,7 +203,7 @@ __global__ void __launch_bounds__(256) k_update(V* Y, const V* X, double2 a, con
 template <typename V>
 void launch_update(V* Y, const V* X, double2 a, const V* W1, const V* W2, double2 w2c, const double2* C,
                    const V* E, double2 e, long long rows, int L, hipStream_t s) {
-  const size_t lds = ((size_t)L * 64 + 16 * (size_t)L) * sizeof(V);
+  const size_t lds = ((size_t)L * 64 + 16 * (size_t)L) * sizeof(double2);
   static bool attr = false;
   if (!attr) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_update<V>),

@@ -28,6 +28,14 @@ __device__ __forceinline__ V cx(typename RealOf<V>::T x, typename RealOf<V>::T y
   v.y = y;
   return v;
 }
+// Guarded load returning a value: `c ? p[i] : zero` with two lvalues would
+// make the compiler load through a pointer select (a flat load that may hit a
+// private copy of zero), so the zero is produced as a value instead.
+template <typename V> __device__ __forceinline__ V load_or_zero(bool c, const V* p) {
+  V v = cx<V>(0, 0);
+  if (c) v = *p;
+  return v;
+}
 template <typename V> __device__ __forceinline__ V cadd(V a, V b) { return cx<V>(a.x + b.x, a.y + b.y); }
 template <typename V> __device__ __forceinline__ V csub(V a, V b) { return cx<V>(a.x - b.x, a.y - b.y); }
 template <typename V> __device__ __forceinline__ V cmul(V a, V b) {
@@ -274,11 +282,10 @@ __global__ void __launch_bounds__(512) k_line_pass(LinePass lp) {
   const int np_here = min(P, lp.npts - pt0);
   const bool active = tm.rt < tm.TPC && p < np_here;
   const int pt = pt0 + p;
-  const V zero = cx<V>(0, 0);
   const V* __restrict__ src = reinterpret_cast<const V*>(lp.src) + (long long)lo * lp.lo_src;
-  for (int i = tid; i < M; i += nthr) tws[i] = reinterpret_cast<const V*>(lp.tw)[i];
 
-  // ---- load: every global load of this thread is issued before any LDS store
+  // ---- load: every global load of this thread (inputs, then twiddles) is
+  // issued before any LDS store
   V v[VPT];
   if (MODE == PASS_SCATTER_Z || MODE == PASS_CUBOID_Z) {
     const int c = q / lp.Lb, l = lo * lp.Lb + q % lp.Lb;
@@ -294,16 +301,19 @@ __global__ void __launch_bounds__(512) k_line_pass(LinePass lp) {
     }
     const V* __restrict__ S = reinterpret_cast<const V*>(lp.S);
 #pragma unroll
-    for (int i = 0; i < VPT; ++i) v[i] = mrow[i] >= 0 ? S[(3 * mrow[i] + c) * lp.L + l] : zero;
+    for (int i = 0; i < VPT; ++i) v[i] = load_or_zero(mrow[i] >= 0, S + (3 * mrow[i] + c) * lp.L + l);
   } else {
     const int a = pt / lp.ptb, b = pt % lp.ptb;
     const V* __restrict__ base = src + a * lp.sa + b * lp.sb + q;
 #pragma unroll
     for (int i = 0; i < VPT; ++i) {
       const int e = tm.rt + i * tm.TPC;
-      v[i] = (active && e < lp.nin) ? base[e * lp.se] : zero;
+      v[i] = load_or_zero(active && e < lp.nin, base + e * lp.se);
     }
   }
+  const V twv = load_or_zero(tid < M, reinterpret_cast<const V*>(lp.tw) + tid);
+  if (tid < M) tws[tid] = twv;
+  for (int i = tid + nthr; i < M; i += nthr) tws[i] = reinterpret_cast<const V*>(lp.tw)[i];
 #pragma unroll
   for (int i = 0; i < VPT; ++i) {
     const int e = tm.rt + i * tm.TPC;
@@ -404,13 +414,49 @@ __device__ __forceinline__ void reg2_step1(V* a, const V* tws, int j2) {
   }
 }
 
-template <typename V, int R1, int R2, int MODE>
+// Entry i (of n) of the x pass's staged Green lines, laid out [p][qx][6] for
+// the points pt0, pt0 + 1, ... (zero for i >= n).  The half spectrum is stored
+// x-innermost, [(ay*hz + az)*hx + ax][6]; the odd components (xy, xz, yz)
+// flip sign under the mirror q -> m - q.
+constexpr int kGreenPerThread = 4;
+template <typename V, int M>
+__device__ __forceinline__ V green_entry(const LinePass& lp, int pt0, int i, int n) {
+  const int comp = i % 6, qx = (i / 6) % M, pp = i / (6 * M);
+  const int ptg = pt0 + pp;
+  const int ky = ptg / lp.mz, kz = ptg - ky * lp.mz;
+  int ax = qx, ay = ky, az = kz;
+  bool sgn = false;
+  if (ax > lp.mx - ax) { ax = lp.mx - ax; sgn ^= (comp == 1 || comp == 2); }
+  if (ay > lp.my - ay) { ay = lp.my - ay; sgn ^= (comp == 1 || comp == 4); }
+  if (az > lp.mz - az) { az = lp.mz - az; sgn ^= (comp == 2 || comp == 4); }
+  const V* __restrict__ G = reinterpret_cast<const V*>(lp.green);
+  const V g = load_or_zero(i < n, G + (((long long)ay * lp.hz + az) * lp.hx + ax) * 6 + comp);
+  return sgn ? cx<V>(-g.x, -g.y) : g;
+}
+
+// Points per workgroup of a register kernel for Q columns per point: as many
+// as 512 threads and a 4096-value tile allow, at least one.
+constexpr int reg2_points(int M, int RMAX, int Q) {
+  return (512 / (Q * RMAX) < 4096 / (M * Q) ? 512 / (Q * RMAX) : 4096 / (M * Q)) < 1
+             ? 1
+             : (512 / (Q * RMAX) < 4096 / (M * Q) ? 512 / (Q * RMAX) : 4096 / (M * Q));
+}
+
+// LB > 0: column tile fixed at LB (Q = 3 LB columns per point, P points per
+// workgroup from reg2_points), so that the tile width and every LDS offset
+// are compile-time constants; LB = 0: tile taken from the launch arguments.
+template <typename V, int R1, int R2, int MODE, int LB>
 __global__ void __launch_bounds__(512) k_reg2(LinePass lp) {
   constexpr int M = R1 * R2;
   constexpr int RMAX = R1 > R2 ? R1 : R2;
   constexpr bool INV_PASS = (MODE == PASS_GATHER_Z || MODE == PASS_CROP_Z || MODE == PASS_PLAIN_INV);
+  constexpr int PB = LB ? reg2_points(M, RMAX, 3 * LB) : 0;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int P = lp.P, Q = lp.Q, QQ = P * Q;
+  const int Lb = LB ? LB : lp.Lb;
+  const int Q = LB ? 3 * LB : lp.Q;
+  const int P = LB ? PB : lp.P;
+  const int QQ = P * Q;
+  const int nthr = LB ? PB * 3 * LB * RMAX : (int)blockDim.x;
   V* buf = reinterpret_cast<V*>(smem_raw);
   V* tws = buf + M * QQ;
   V* gls = tws + M;  // Green tensor of the tile's points: [p][qx][6]
@@ -431,35 +477,17 @@ __global__ void __launch_bounds__(512) k_reg2(LinePass lp) {
   const int np_here = min(P, lp.npts - pt0);
   const bool active = p < np_here;
   const int pt = pt0 + p;
-  const V zero = cx<V>(0, 0);
   const bool s1 = w < R2, s2 = w < R1;
   const int j2 = w, k1 = w;
-  for (int i = tid; i < M; i += blockDim.x) tws[i] = reinterpret_cast<const V*>(lp.tw)[i];
-  if (MODE == PASS_CONV_X) {
-    // half spectrum stored x-innermost: [(ay*hz + az)*hx + ax][6]; the
-    // odd components (xy, xz, yz) flip sign under the mirror q -> m - q
-    const V* __restrict__ G = reinterpret_cast<const V*>(lp.green);
-    for (int i = tid; i < np_here * M * 6; i += blockDim.x) {
-      const int comp = i % 6, qx = (i / 6) % M, pp = i / (6 * M);
-      const int ptg = pt0 + pp;
-      const int ky = ptg / lp.mz, kz = ptg % lp.mz;
-      int ax = qx, ay = ky, az = kz;
-      int sgn = 0;
-      if (ax > lp.mx - ax) { ax = lp.mx - ax; sgn ^= (comp == 1 || comp == 2); }
-      if (ay > lp.my - ay) { ay = lp.my - ay; sgn ^= (comp == 1 || comp == 4); }
-      if (az > lp.mz - az) { az = lp.mz - az; sgn ^= (comp == 2 || comp == 4); }
-      V g = G[(((long long)ay * lp.hz + az) * lp.hx + ax) * 6 + comp];
-      if (sgn) g = cx<V>(-g.x, -g.y);
-      gls[i] = g;
-    }
-  }
   {
     const int lo = lo_blk;
-    // ---- step 1 input from global ---------------------------------------------
+    // ---- global loads, all issued before the first LDS store so that their
+    // latencies overlap: the point's inputs, the x pass's Green lines, and
+    // the twiddle table ----------------------------------------------------------
     V a[RMAX];
     const V* __restrict__ src = reinterpret_cast<const V*>(lp.src) + (long long)lo * lp.lo_src;
     if (MODE == PASS_SCATTER_Z || MODE == PASS_CUBOID_Z) {
-      const int c = q / lp.Lb, l = lo * lp.Lb + q % lp.Lb;
+      const int c = q / Lb, l = lo * Lb + q % Lb;
       long long mrow[R1];
 #pragma unroll
       for (int j1 = 0; j1 < R1; ++j1) {
@@ -472,17 +500,46 @@ __global__ void __launch_bounds__(512) k_reg2(LinePass lp) {
       }
       const V* __restrict__ S = reinterpret_cast<const V*>(lp.S);
 #pragma unroll
-      for (int j1 = 0; j1 < R1; ++j1) a[j1] = mrow[j1] >= 0 ? S[(3 * mrow[j1] + c) * lp.L + l] : zero;
+      for (int j1 = 0; j1 < R1; ++j1) a[j1] = load_or_zero(mrow[j1] >= 0, S + (3 * mrow[j1] + c) * lp.L + l);
     } else {
       const int ga = pt / lp.ptb, gb = pt % lp.ptb;
-      const V* __restrict__ base = src + ga * lp.sa + gb * lp.sb + q;
+      const int se = (int)lp.se;  // a line spans fewer than 2^31 elements (checked at launch)
+      const V* __restrict__ base = src + ga * lp.sa + gb * lp.sb + q + j2 * se;
 #pragma unroll
       for (int j1 = 0; j1 < R1; ++j1) {
         const int e = j1 * R2 + j2;
-        a[j1] = (s1 && active && e < lp.nin) ? base[e * lp.se] : zero;
+        a[j1] = load_or_zero(s1 && active && e < lp.nin, base + j1 * (R2 * se));
       }
     }
-    __syncthreads();  // staged tables ready; previous tile's LDS reads done
+    // Green lines of the tile's points (x pass), kGreenPerThread per thread in
+    // registers; the launcher keeps np * M * 6 within kGreenPerThread * nthr
+    const int ng = MODE == PASS_CONV_X ? np_here * M * 6 : 0;
+    V gv[kGreenPerThread];
+    if (MODE == PASS_CONV_X) {
+#pragma unroll
+      for (int u = 0; u < kGreenPerThread; ++u) gv[u] = green_entry<V, M>(lp, pt0, tid + u * nthr, ng);
+    }
+    // z-inverse with gather: occupied index of each output cell this thread
+    // stores (-1 for empty cells and cropped elements)
+    int outm[MODE == PASS_GATHER_Z ? R2 : 1];
+    if (MODE == PASS_GATHER_Z) {
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) {
+        const int e = k1 + R1 * k2;
+        const bool ok = s2 && active && e < lp.nout;
+        outm[k2] = ok ? lp.inv[(long long)pt * lp.nz + e] : -1;
+      }
+    }
+    const V twv = load_or_zero(tid < M, reinterpret_cast<const V*>(lp.tw) + tid);
+    if (tid < M) tws[tid] = twv;
+    for (int i = tid + nthr; i < M; i += nthr) tws[i] = reinterpret_cast<const V*>(lp.tw)[i];
+    if (MODE == PASS_CONV_X) {
+#pragma unroll
+      for (int u = 0; u < kGreenPerThread; ++u)
+        if (tid + u * nthr < ng) gls[tid + u * nthr] = gv[u];
+      for (int i = tid + kGreenPerThread * nthr; i < ng; i += nthr) gls[i] = green_entry<V, M>(lp, pt0, i, ng);
+    }
+    __syncthreads();  // staged tables ready
     if (s1) {
       reg2_step1<R1, R2, INV_PASS>(a, tws, j2);
 #pragma unroll
@@ -503,8 +560,7 @@ __global__ void __launch_bounds__(512) k_reg2(LinePass lp) {
         for (int k2 = 0; k2 < R2; ++k2) buf[(k1 + R1 * k2) * QQ + qq] = a[k2];
       }
       __syncthreads();
-      const int Lb = lp.Lb;
-      for (int t = tid; t < np_here * Lb * M; t += blockDim.x) {
+      for (int t = tid; t < np_here * Lb * M; t += nthr) {
         const int li = t % Lb, kx = (t / Lb) % M, pp = t / (Lb * M);
         const V* g = gls + (pp * M + kx) * 6;
         V* row = buf + kx * QQ + pp * Q;
@@ -536,51 +592,54 @@ __global__ void __launch_bounds__(512) k_reg2(LinePass lp) {
       if (MODE == PASS_GATHER_Z || MODE == PASS_CROP_Z) {
         using T = typename RealOf<V>::T;
         const T sc = (T)lp.scale;
-        const int c = q / lp.Lb, l = lo * lp.Lb + q % lp.Lb;
+        const int c = q / Lb, l = lo * Lb + q % Lb;
         if (l < lp.L) {
           V* __restrict__ out = reinterpret_cast<V*>(lp.AS);
-          long long row[R2];
+          // rows 3m + c of the output cells (m from the prefetched index)
+          int row[R2];
 #pragma unroll
           for (int k2 = 0; k2 < R2; ++k2) {
             const int e = k1 + R1 * k2;
-            row[k2] = -1;
-            if (e < lp.nout) {
-              const long long cell = (long long)pt * lp.nz + e;
-              const long long m = (MODE == PASS_GATHER_Z) ? (long long)lp.inv[cell] : cell;
-              if (m >= 0) row[k2] = 3 * m + c;
-            }
+            const int m = (MODE == PASS_GATHER_Z) ? outm[k2] : (e < lp.nout ? pt * lp.nz + e : -1);
+            row[k2] = m >= 0 ? 3 * m + c : -1;
           }
           if (MODE == PASS_GATHER_Z) {
             const V* __restrict__ S = reinterpret_cast<const V*>(lp.S);
             const V* __restrict__ dg = reinterpret_cast<const V*>(lp.diag);
             const V* __restrict__ jp = reinterpret_cast<const V*>(lp.jpre);
-            V sv[R2], dv[R2], jv[R2];
+            // groups of four outputs keep the staged operands within budget
+            constexpr int H = R2 < 4 ? R2 : 4;
 #pragma unroll
-            for (int k2 = 0; k2 < R2; ++k2) {
-              if (row[k2] >= 0) {
-                sv[k2] = S[row[k2] * lp.L + l];
-                dv[k2] = dg[row[k2]];
-                jv[k2] = jp[row[k2]];
+            for (int k0 = 0; k0 < R2; k0 += H) {
+              V sv[H], dv[H], jv[H];
+#pragma unroll
+              for (int h = 0; h < H && k0 + h < R2; ++h) {
+                const int r = row[k0 + h];
+                sv[h] = load_or_zero(r >= 0, S + (long long)r * lp.L + l);
+                dv[h] = load_or_zero(r >= 0, dg + r);
+                jv[h] = load_or_zero(r >= 0, jp + r);
+              }
+#pragma unroll
+              for (int h = 0; h < H && k0 + h < R2; ++h) {
+                const int r = row[k0 + h];
+                if (r >= 0) out[(long long)r * lp.L + l] = cmul(jv[h], cadd(cmul(dv[h], sv[h]), cscale(a[k0 + h], sc)));
               }
             }
-#pragma unroll
-            for (int k2 = 0; k2 < R2; ++k2)
-              if (row[k2] >= 0)
-                out[row[k2] * lp.L + l] = cmul(jv[k2], cadd(cmul(dv[k2], sv[k2]), cscale(a[k2], sc)));
           } else {
 #pragma unroll
             for (int k2 = 0; k2 < R2; ++k2)
-              if (row[k2] >= 0) out[row[k2] * lp.L + l] = cscale(a[k2], sc);
+              if (row[k2] >= 0) out[(long long)row[k2] * lp.L + l] = cscale(a[k2], sc);
           }
         }
       } else {
         V* __restrict__ dst = reinterpret_cast<V*>(lp.dst) + (long long)lo * lp.lo_dst;
         const int ga = pt / lp.ptb, gb = pt % lp.ptb;
-        V* __restrict__ obase = dst + ga * lp.da + gb * lp.db + q;
+        const int de = (int)lp.de;
+        V* __restrict__ obase = dst + ga * lp.da + gb * lp.db + q + k1 * de;
 #pragma unroll
         for (int k2 = 0; k2 < R2; ++k2) {
           const int e = k1 + R1 * k2;
-          if (e < lp.nout) obase[e * lp.de] = a[k2];
+          if (e < lp.nout) obase[k2 * (R1 * de)] = a[k2];
         }
       }
     }
@@ -602,28 +661,56 @@ static void launch_mode(const LinePass& lp, dim3 grid, int nthr, size_t lds, hip
   hipLaunchKernelGGL((k_line_pass<V, kVPT, MODE>), grid, dim3(nthr), lds, s, lp);
 }
 
-template <typename V, int R1, int R2, int MODE>
+template <typename V, int R1, int R2, int MODE, int LB>
 static void launch_reg2_mode(const LinePass& lp, dim3 grid, int nthr, size_t lds, hipStream_t s) {
   static bool attr_set = false;
   if (!attr_set) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_reg2<V, R1, R2, MODE>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_reg2<V, R1, R2, MODE, LB>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     attr_set = true;
   }
-  hipLaunchKernelGGL((k_reg2<V, R1, R2, MODE>), grid, dim3(nthr), lds, s, lp);
+  hipLaunchKernelGGL((k_reg2<V, R1, R2, MODE, LB>), grid, dim3(nthr), lds, s, lp);
 }
+
+template <typename V, int R1, int R2, int LB>
+static void launch_reg2_tile(const LinePass& lp, dim3 grid, int nthr, size_t lds, hipStream_t s) {
+  const int mode = (lp.mode == PASS_PLAIN && lp.inverse) ? PASS_PLAIN_INV : lp.mode;
+  switch (mode) {
+    case PASS_PLAIN: launch_reg2_mode<V, R1, R2, PASS_PLAIN, LB>(lp, grid, nthr, lds, s); break;
+    case PASS_PLAIN_INV: launch_reg2_mode<V, R1, R2, PASS_PLAIN_INV, LB>(lp, grid, nthr, lds, s); break;
+    case PASS_SCATTER_Z: launch_reg2_mode<V, R1, R2, PASS_SCATTER_Z, LB>(lp, grid, nthr, lds, s); break;
+    case PASS_CONV_X: launch_reg2_mode<V, R1, R2, PASS_CONV_X, LB>(lp, grid, nthr, lds, s); break;
+    case PASS_GATHER_Z: launch_reg2_mode<V, R1, R2, PASS_GATHER_Z, LB>(lp, grid, nthr, lds, s); break;
+    case PASS_CUBOID_Z: launch_reg2_mode<V, R1, R2, PASS_CUBOID_Z, LB>(lp, grid, nthr, lds, s); break;
+    default: launch_reg2_mode<V, R1, R2, PASS_CROP_Z, LB>(lp, grid, nthr, lds, s); break;
+  }
+}
+
+// Column tile with a compile-time specialisation (8 columns per component,
+// the tile of L >= 8 on lengths up to 170).
+constexpr int kFixedLb = 8;
 
 template <typename V, int R1, int R2>
 static void launch_reg2(LinePass lp, hipStream_t s) {
   constexpr int M = R1 * R2, RMAX = R1 > R2 ? R1 : R2;
   const bool conv = lp.mode == PASS_CONV_X;
+  const bool fixed = lp.Lb == kFixedLb && lp.Q == 3 * kFixedLb;
   int P = std::min(512 / (lp.Q * RMAX), 4096 / (M * lp.Q));
   if (P < 1) P = 1;
-  if (P > lp.npts) P = lp.npts;
+  if (fixed) {
+    P = reg2_points(M, RMAX, 3 * kFixedLb);
+  } else if (P > lp.npts) {
+    P = lp.npts;
+  }
   lp.P = P;
   const int nthr = P * lp.Q * RMAX;
   if (nthr > 512) {
     set_error("register FFT launch exceeds 512 threads");
+    return;
+  }
+  // element offsets along a line are 32-bit in the kernel
+  if ((long long)M * (lp.se > lp.de ? lp.se : lp.de) >= (1LL << 31)) {
+    set_error("register FFT line stride exceeds 32-bit element offsets");
     return;
   }
   const size_t lds = ((size_t)M * P * lp.Q + M + (conv ? (size_t)P * M * 6 : 0)) * sizeof(V);
@@ -631,16 +718,10 @@ static void launch_reg2(LinePass lp, hipStream_t s) {
   const int nlo = lp.nlo > 0 ? lp.nlo : 1;
   dim3 grid(nblk, nlo);
   if (conv) grid = dim3(((nblk + 7) / 8) * 8 * nlo, 1);
-  const int mode = (lp.mode == PASS_PLAIN && lp.inverse) ? PASS_PLAIN_INV : lp.mode;
-  switch (mode) {
-    case PASS_PLAIN: launch_reg2_mode<V, R1, R2, PASS_PLAIN>(lp, grid, nthr, lds, s); break;
-    case PASS_PLAIN_INV: launch_reg2_mode<V, R1, R2, PASS_PLAIN_INV>(lp, grid, nthr, lds, s); break;
-    case PASS_SCATTER_Z: launch_reg2_mode<V, R1, R2, PASS_SCATTER_Z>(lp, grid, nthr, lds, s); break;
-    case PASS_CONV_X: launch_reg2_mode<V, R1, R2, PASS_CONV_X>(lp, grid, nthr, lds, s); break;
-    case PASS_GATHER_Z: launch_reg2_mode<V, R1, R2, PASS_GATHER_Z>(lp, grid, nthr, lds, s); break;
-    case PASS_CUBOID_Z: launch_reg2_mode<V, R1, R2, PASS_CUBOID_Z>(lp, grid, nthr, lds, s); break;
-    default: launch_reg2_mode<V, R1, R2, PASS_CROP_Z>(lp, grid, nthr, lds, s); break;
-  }
+  if (fixed)
+    launch_reg2_tile<V, R1, R2, kFixedLb>(lp, grid, nthr, lds, s);
+  else
+    launch_reg2_tile<V, R1, R2, 0>(lp, grid, nthr, lds, s);
 }
 
 // Lengths with a two-step register kernel (m = R1 * R2); others use the

@@ -350,7 +350,20 @@ struct Ctx {
   long long rows;
   hipStream_t s;
   std::vector<double>* hist;
+  // inner complex64 solves of the mixed mode end early once they stall, so
+  // that the float64 refinement restarts them from the true residual
+  bool stop_on_stall = false;
 };
+
+// The last two residuals both failed to improve on the best one before them.
+static bool stalled(const Ctx& c) {
+  if (!c.stop_on_stall || !c.hist || c.hist->size() < 3) return false;
+  const std::vector<double>& h = *c.hist;
+  const size_t k = h.size() - 1;
+  double best = h[0];
+  for (size_t i = 1; i + 1 < k; ++i) best = std::min(best, h[i]);
+  return h[k - 1] >= best && h[k] >= best;
+}
 
 static int gram_to_host(Ctx& c, const void* A, const void* B, bool conj, bool f32, std::vector<zc>& out) {
   bdda_plan* p = c.p;
@@ -514,7 +527,7 @@ static int solve_bicgstab(Ctx& c, const V* Bj, V* X, bool x_zero, double bnorm, 
     err = std::sqrt(rn2) / bnorm;
     if (c.hist) c.hist->push_back(err);
     it = k;
-    if (err < tol) break;
+    if (err < tol || stalled(c)) break;
     if ((rc = gram_to_host(c, R0, Z, true, is_f32<V>(), RZ))) return rc;
     beta = RZ;
     for (auto& v : beta) v = -v;
@@ -593,7 +606,7 @@ static int solve_cocg(Ctx& c, const V* Bj, V* X, bool x_zero, double bnorm, doub
     err = la::frob(xi) / bnorm;
     if (c.hist) c.hist->push_back(err);
     it = k;
-    if (err < tol) break;
+    if (err < tol || stalled(c)) break;
     if ((rc = gram_to_host(c, Qn, Qn, false, is_f32<V>(), QntQn))) return rc;
     la::matmul(tau, true, QntQn, rhs, L);  // tau^T (Qn^T Qn)
     LU = QtQ;
@@ -976,11 +989,13 @@ int bdda_solve(bdda_plan* p, int solver, int precision, const void* B_dev, const
       if (inner_tol < 2e-6) inner_tol = 2e-6;
       std::vector<double> inner;
       c.hist = &inner;
+      c.stop_on_stall = true;
       int iit = 0;
       double ierr = 0.0;
       const int budget = std::min(cycle, itermax - total);
       if ((rc = run_solver<float2>(c, solver, R32, D32, true, 1.0, inner_tol, budget, ws, &iit, &ierr))) return rc;
       c.hist = &hist;
+      c.stop_on_stall = false;
       for (double e : inner) hist.push_back(e * rs / bnorm);
       total += (int)inner.size();
       launch_add_c64_to_c128(X, D32, n, rs, s);
