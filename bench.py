@@ -7,7 +7,7 @@ beta = 0.2 (seed 0) on a 51 x 48 x 63 lattice padded to a 64^3 cuboid
 m_m = 1.33, L = 64 orientations, complex128 block-BiCGSTAB to relative
 residual 1e-5 with B resident on the device.  Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-mixed] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-mixed] [--no-batch] [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -106,6 +106,53 @@ def cpu_baseline(geometry, wl, gpu_iters, sample_L=16):
                       f"{wl['L'] // sample_L} in L and x{gpu_iters} iterations (the GPU float64 count)"}
 
 
+def batch_leg(torch, nstreams=8):
+    """Batched uneven instances (32 GRE particles, lattices 20..48 cells,
+    L = 16, mixed block-COCG-rq): one-at-a-time solves against one
+    concurrent batch on `nstreams` HIP streams, both after a warm-up."""
+    import numpy as np
+    from bdda_amd import workloads
+    from bdda_amd.batch import solve_batch
+    wl = workloads.BATCH_32
+    insts = workloads.batch_instances(wl)
+    t0 = time.time()
+    geos = [workloads.build_geometry(w) for w in insts]
+    t_geo = time.time() - t0
+    dds = [workloads.make_dipoles(w, geometry=g, precision=wl["precision"]) for w, g in zip(insts, geos)]
+    for dd in dds:
+        dd.solve_matrix_equation()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for dd in dds:
+        dd.solve_matrix_equation()
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    X_seq = [dd.X_device.clone() for dd in dds]
+    solve_batch(dds, nstreams)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solve_batch(dds, nstreams)
+    torch.cuda.synchronize()
+    t_bat = time.perf_counter() - t0
+    n_or = sum(dd.L for dd in dds)
+    N = [int(dd.num_element_occupy) for dd in dds]
+    its = [dd.iter_fin + 1 for dd in dds]
+    return {
+        "workload": wl["name"], "instances": len(dds), "value": round(n_or / t_bat, 2), "unit": "orientations/s",
+        "orientations": n_or, "batched_s": round(t_bat, 4), "sequential_s": round(t_seq, 4),
+        "sequential_value": round(n_or / t_seq, 2), "speedup_vs_sequential": round(t_seq / t_bat, 3),
+        "nstreams": nstreams, "precision": wl["precision"], "solver": wl["solver"],
+        "identical_to_sequential": all(torch.equal(dd.X_device, x) for dd, x in zip(dds, X_seq)),
+        "all_converged": all(dd.converge for dd in dds),
+        "max_true_residual": max(dd.true_residual for dd in dds),
+        "dipoles": {"min": min(N), "max": max(N), "total": sum(N)},
+        "longest_axis": {"min": int(min(max(dd.lattice_n) for dd in dds)),
+                         "max": int(max(max(dd.lattice_n) for dd in dds))},
+        "iterations": {"min": min(its), "max": max(its), "mean": round(float(np.mean(its)), 2)},
+        "geometry_s": round(t_geo, 1),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +160,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU layout is a design note, "
@@ -193,6 +241,12 @@ def main():
             "passes_c64": mp,
         }
         log(f"mixed: {dtm / a.steps * 1e3:.1f} ms/solve")
+
+    if not a.no_batch:
+        log("batched uneven instances")
+        result["batch"] = batch_leg(torch)
+        log(f"batch: {result['batch']['value']} orientations/s, x{result['batch']['speedup_vs_sequential']} "
+            "vs one at a time")
 
     if not a.no_cpu_baseline:
         log("cpu baseline")

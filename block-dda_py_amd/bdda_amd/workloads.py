@@ -27,6 +27,44 @@ SMALL_24 = dict(
 )
 
 
+# Batched uneven instances: 32 GRE particles (shape seeds 0-31) whose longest
+# lattice axis is drawn uniformly from 20..48 cells, bc and ab uniform in
+# [0.5, 1], beta uniform in [0, 0.3], isotropic m_p with real part uniform in
+# [1.4, 1.7] and imaginary part log-uniform in [1e-4, 1e-1], m_m = 1,
+# 15 dipoles per wavelength, L = 16 orientations each (orientation seed =
+# shape seed + 1); mixed-precision block-COCG-rq to 1e-5.
+BATCH_32 = dict(
+    name="batch32_gre_grid20to48_L16", count=32, grid=(20, 48), bc=(0.5, 1.0), ab=(0.5, 1.0), beta=(0.0, 0.3),
+    re_m=(1.4, 1.7), im_m=(1e-4, 1e-1), m_m=1.0, dpl=15.0, L=16, solver="cocg_rq", precision="mixed", tol=1e-5,
+    itermax=500,
+)
+
+
+def rv_for_grid(cells, bc, ab):
+    """r_v whose beta = 0 ellipsoid spans about `cells` lattice cells along
+    its longest axis: the axis is 2c, the spacing (1/75) sqrt(c/0.05)
+    (ab bc)^(1/5), and c = cbrt(r_v^3 / (ab bc^2))."""
+    c = ((cells - 2) * (ab * bc) ** 0.2 / 150.0) ** 2 / 0.05
+    return float(np.cbrt(c ** 3 * ab * bc ** 2))
+
+
+def batch_instances(wl, count=None):
+    """Per-instance workload dicts of a batched workload: instance i draws its
+    size, shape and material from default_rng(i) and uses shape seed i."""
+    out = []
+    for i in range(wl["count"] if count is None else count):
+        r = np.random.default_rng(i)
+        cells = int(r.integers(wl["grid"][0], wl["grid"][1] + 1))
+        bc, ab = r.uniform(*wl["bc"]), r.uniform(*wl["ab"])
+        beta = r.uniform(*wl["beta"])
+        m = r.uniform(*wl["re_m"]) + 1j * np.exp(r.uniform(np.log(wl["im_m"][0]), np.log(wl["im_m"][1])))
+        out.append(dict(name=f"{wl['name']}_{i}", r_v_base=rv_for_grid(cells, bc, ab), bc_ratio=bc, ab_ratio=ab,
+                        beta=beta, shape_seed=i, m_p_xyz=(m,) * 3, m_m=wl["m_m"], dpl=wl["dpl"], L=wl["L"],
+                        orient_seed=i + 1, pad_to=None, solver=wl["solver"], tol=wl["tol"],
+                        itermax=wl["itermax"]))
+    return out
+
+
 def euler_angles(L, seed):
     rng = np.random.default_rng(seed)
     return np.column_stack([rng.uniform(0, 2 * np.pi, L), rng.uniform(0, np.pi, L), rng.uniform(0, 2 * np.pi, L)])

@@ -176,14 +176,20 @@ class DiscreteDipoles(Target, IncidentField):
     def solve_matrix_equation(self):
         """Device block-Krylov solve (scatterer.py:164-184)."""
         start = time.time()
-        name = "block-BiCGStab" if self.solver == "bicgstab" else "block-COCG-rq"
         if self.verbose:
-            print("Starting {} iterative solver...".format(name))
+            print("Starting {} iterative solver...".format(self._solver_name()))
         X, it, err, tr, hist, warm = self._plan.solve(self._B_dev, self.solver, self.precision, self.tol,
                                                       self.itermax, X0=self.X0, refine_every=self.refine_every)
+        self._store_solution(X, it, err, tr, hist, warm, time.time() - start)
+
+    def _solver_name(self):
+        return "block-BiCGStab" if self.solver == "bicgstab" else "block-COCG-rq"
+
+    def _store_solution(self, X, it, err, tr, hist, warm, seconds):
+        """Solver outputs -> attributes, reporting as scatterer.py:173-184."""
         if not hist:
             err = float('inf')
-        self.solve_time = time.time() - start
+        self.solve_time = seconds
         self.iter_fin, self.err_fin, self.true_residual = it, err, tr
         self.err_history, self.warm_start_used = hist, warm
         if self.verbose:
@@ -191,11 +197,10 @@ class DiscreteDipoles(Target, IncidentField):
                 print("iter= {:}, err= {:.4f}".format(k, e))
         self._X_dev = X
         self._obs = None
-        if err < self.tol:
-            self.converge = True
-            if self.verbose:
-                print("{} converged! (iter_fin={:}, err_fin={:.4f}, solver time={:.1f}s)".format(
-                    name, it, err, self.solve_time))
+        self.converge = err < self.tol
+        if self.converge and self.verbose:
+            print("{} converged! (iter_fin={:}, err_fin={:.4f}, solver time={:.1f}s)".format(
+                self._solver_name(), it, err, self.solve_time))
 
     @property
     def X(self):

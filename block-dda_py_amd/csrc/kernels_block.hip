@@ -4,6 +4,8 @@
 // Krylov blocks are (rows, L) row-major with rows = 3N (row = 3m + c) and the
 // L orientation columns innermost, so a wavefront reads 64 consecutive
 // columns of one row.  Small L x L coefficient matrices are complex float64.
+#include <mutex>
+
 #include "internal.h"
 
 namespace bdda {
@@ -204,12 +206,11 @@ template <typename V>
 void launch_update(V* Y, const V* X, double2 a, const V* W1, const V* W2, double2 w2c, const double2* C,
                    const V* E, double2 e, long long rows, int L, hipStream_t s) {
   const size_t lds = ((size_t)L * 64 + 16 * (size_t)L) * sizeof(double2);
-  static bool attr = false;
-  if (!attr) {
+  static std::once_flag attr_once;
+  std::call_once(attr_once, [] {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_update<V>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    attr = true;
-  }
+  });
   long long slabs = (rows + 15) / 16;
   int g = (int)min(slabs, 4096LL);
   dim3 grid(g, (L + 63) / 64);

@@ -12,6 +12,8 @@
 //   T2  [lo][ix][ky][kz][c][li]   (Nx, My, Mz, 3, Lb)  after the y pass
 // The x pass (forward x, 3x3 Green multiply, inverse x) runs in place on T2.
 #include <algorithm>
+#include <atomic>
+#include <mutex>
 
 #include "internal.h"
 
@@ -648,27 +650,25 @@ __global__ void __launch_bounds__(512) k_reg2(LinePass lp) {
 
 constexpr int kVPT = 8;
 constexpr int kTile = 4096;  // complex values per workgroup tile
-static bool g_force_generic = false;
+static std::atomic<bool> g_force_generic{false};
 
 template <typename V, int MODE>
 static void launch_mode(const LinePass& lp, dim3 grid, int nthr, size_t lds, hipStream_t s) {
-  static bool attr_set = false;
-  if (!attr_set) {
+  static std::once_flag attr_once;
+  std::call_once(attr_once, [] {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_line_pass<V, kVPT, MODE>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    attr_set = true;
-  }
+  });
   hipLaunchKernelGGL((k_line_pass<V, kVPT, MODE>), grid, dim3(nthr), lds, s, lp);
 }
 
 template <typename V, int R1, int R2, int MODE, int LB>
 static void launch_reg2_mode(const LinePass& lp, dim3 grid, int nthr, size_t lds, hipStream_t s) {
-  static bool attr_set = false;
-  if (!attr_set) {
+  static std::once_flag attr_once;
+  std::call_once(attr_once, [] {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_reg2<V, R1, R2, MODE, LB>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    attr_set = true;
-  }
+  });
   hipLaunchKernelGGL((k_reg2<V, R1, R2, MODE, LB>), grid, dim3(nthr), lds, s, lp);
 }
 

@@ -6,12 +6,14 @@
 // (and a few scalars) travel to the host, where the float64 coefficient
 // solves, Cholesky-QR factors and the convergence test run.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <complex>
 #include <cstdio>
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/bdda.h"
@@ -548,7 +550,20 @@ static int cholqr2(Ctx& c, const V* W, V* Q1, V* Qn, std::vector<zc>& tau) {
   std::vector<zc> G, R1, R1i, R2, R2i;
   int rc;
   if ((rc = gram_to_host(c, W, W, true, is_f32<V>(), G))) return rc;
-  if (!la::cholesky_upper(G, R1, L)) return singular("block QR (Cholesky of W^H W)");
+  bool shifted = false;
+  if (!la::cholesky_upper(G, R1, L)) {
+    // Nearly rank-deficient block (the Gram squares its condition number):
+    // shifted CholeskyQR3 -- factor W^H W + s I with
+    // s = 11 (m n + n (n + 1)) u tr(W^H W), which always succeeds for
+    // cond(W) below 1/u, then two unshifted passes restore orthogonality.
+    double tr = 0.0;
+    for (int i = 0; i < L; ++i) tr += G[(size_t)i * L + i].real();
+    const double u = std::ldexp(1.0, -53);
+    const double shift = 11.0 * ((double)c.rows * L + (double)L * (L + 1)) * u * tr;
+    for (int i = 0; i < L; ++i) G[(size_t)i * L + i] += shift;
+    if (!(tr > 0.0) || !la::cholesky_upper(G, R1, L)) return singular("block QR (Cholesky of W^H W)");
+    shifted = true;
+  }
   la::upper_inverse(R1, R1i, L);
   if ((rc = update<V>(c, Q1, nullptr, 0.0, W, nullptr, 0.0, R1i, 3, nullptr, 0.0))) return rc;
   if ((rc = gram_to_host(c, Q1, Q1, true, is_f32<V>(), G))) return rc;
@@ -556,6 +571,16 @@ static int cholqr2(Ctx& c, const V* W, V* Q1, V* Qn, std::vector<zc>& tau) {
   la::upper_inverse(R2, R2i, L);
   if ((rc = update<V>(c, Qn, nullptr, 0.0, Q1, nullptr, 0.0, R2i, 4, nullptr, 0.0))) return rc;
   la::matmul(R2, false, R1, tau, L);
+  if (shifted) {
+    std::vector<zc> R3, R3i, t2;
+    if ((rc = gram_to_host(c, Qn, Qn, true, is_f32<V>(), G))) return rc;
+    if (!la::cholesky_upper(G, R3, L)) return singular("block QR (Cholesky of Q2^H Q2)");
+    la::upper_inverse(R3, R3i, L);
+    if ((rc = update<V>(c, Q1, nullptr, 0.0, Qn, nullptr, 0.0, R3i, 4, nullptr, 0.0))) return rc;
+    BDDA_HIP(hipMemcpyAsync(Qn, Q1, (size_t)c.rows * L * sizeof(V), hipMemcpyDeviceToDevice, c.s));
+    la::matmul(R3, false, tau, t2, L);
+    tau.swap(t2);
+  }
   return 0;
 }
 
@@ -1025,6 +1050,67 @@ int bdda_solve(bdda_plan* p, int solver, int precision, const void* B_dev, const
   if (err_hist)
     for (size_t i = 0; i < hist.size() && (int)i < std::max(itermax, 1); ++i) err_hist[i] = hist[i];
   if (flags_out) *flags_out = flags | ((int)std::min<size_t>(hist.size(), 0x7fffff) << 8);
+  return 0;
+}
+
+int bdda_solve_batch(int nb, bdda_plan* const* plans, int solver, int precision, const void* const* B,
+                     const void* const* X0, void* const* X, const int* L, double tol, int itermax, int refine_every,
+                     int nstreams, int* iter_fin, double* err_fin, double* true_res, double* err_hist,
+                     int* flags_out) {
+  g_err.clear();
+  if (nb < 1 || !plans || !B || !X || !L || nstreams < 1) {
+    set_error("bdda_solve_batch: need nb >= 1 instances, plans, B, X, L and nstreams >= 1");
+    return BDDA_E_ARG;
+  }
+  std::vector<const bdda_plan*> seen(plans, plans + nb);
+  std::sort(seen.begin(), seen.end());
+  if (std::adjacent_find(seen.begin(), seen.end()) != seen.end() || seen.front() == nullptr) {
+    set_error("bdda_solve_batch: every instance needs its own (non-null) plan");
+    return BDDA_E_ARG;
+  }
+  // largest instances first: the workers take them in this order
+  std::vector<int> order(nb);
+  for (int i = 0; i < nb; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return (double)plans[a]->N * L[a] > (double)plans[b]->N * L[b];
+  });
+  const int nthreads = std::min(std::min(nstreams, nb), 64);
+  int device = 0;
+  BDDA_HIP(hipGetDevice(&device));
+  std::vector<hipStream_t> streams(nthreads, nullptr);
+  for (int t = 0; t < nthreads; ++t) {
+    if (hipStreamCreateWithFlags(&streams[t], hipStreamNonBlocking) != hipSuccess) {
+      for (int u = 0; u < t; ++u) (void)hipStreamDestroy(streams[u]);
+      set_error("bdda_solve_batch: hipStreamCreateWithFlags failed");
+      return BDDA_E_HIP;
+    }
+  }
+  std::atomic<int> next{0};
+  std::vector<int> rcs(nb, 0);
+  std::vector<std::string> errs(nb);
+  const int hstride = std::max(itermax, 1);
+  auto worker = [&](int t) {
+    (void)hipSetDevice(device);
+    for (int k = next++; k < nb; k = next++) {
+      const int i = order[k];
+      rcs[i] = bdda_solve(plans[i], solver, precision, B[i], X0 ? X0[i] : nullptr, X[i], L[i], tol, itermax,
+                          refine_every, iter_fin ? iter_fin + i : nullptr, err_fin ? err_fin + i : nullptr,
+                          true_res ? true_res + i : nullptr, err_hist ? err_hist + (size_t)i * hstride : nullptr,
+                          flags_out ? flags_out + i : nullptr, streams[t]);
+      if (rcs[i]) errs[i] = bdda_last_error();
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nthreads; ++t) pool.emplace_back(worker, t);
+  worker(0);
+  for (auto& th : pool) th.join();
+  for (int t = 0; t < nthreads; ++t) (void)hipStreamDestroy(streams[t]);
+  for (int i = 0; i < nb; ++i) {
+    if (rcs[i]) {
+      set_error("bdda_solve_batch: instance " + std::to_string(i) + ": " + errs[i]);
+      return rcs[i];
+    }
+  }
   return 0;
 }
 

@@ -114,6 +114,22 @@ int bdda_solve(bdda_plan* plan, int solver, int precision, const void* B_dev, co
                int L, double tol, int itermax, int refine_every, int* iter_fin, double* err_fin,
                double* true_res, double* err_hist, int* flags_out, void* stream);
 
+/* Batched uneven instances: solves nb independent problems, instance i on
+ * plans[i] (its own plan, with its own FFT lengths, column tile and Green
+ * spectrum) with B[i] / X0[i] (nullable array or entries) / X[i] of shape
+ * (3 N_i, L[i]) complex128 on the device.  nstreams host threads, each with
+ * its own non-blocking HIP stream, take the instances largest first and run
+ * bdda_solve on them concurrently; the per-instance results go to
+ * iter_fin[i], err_fin[i], true_res[i], flags_out[i] and
+ * err_hist[i * max(itermax, 1) ...] (each array nullable).  Returns after
+ * every instance has finished; the first failing instance's error is
+ * reported.  No counterpart in the reference (its solver takes one particle
+ * at a time); a batch gives the same per-instance results as bdda_solve. */
+int bdda_solve_batch(int nb, bdda_plan* const* plans, int solver, int precision, const void* const* B,
+                     const void* const* X0, void* const* X, const int* L, double tol, int itermax, int refine_every,
+                     int nstreams, int* iter_fin, double* err_fin, double* true_res, double* err_hist,
+                     int* flags_out);
+
 /* Per-pass kernel timing of the FFT operator with HIP events recorded on
  * the launch stream around every pass (z, y, x-convolution, inverse y,
  * inverse z; complex128 then complex64), for measurement only.

@@ -256,6 +256,28 @@ def test_warm_start_and_fallback():
     assert rel(dd2.X, dd0.X) < 1e-12
 
 
+def test_cocg_nearly_dependent_orientations():
+    """Two orientations 1e-9 rad apart make the right-hand-side block nearly
+    rank deficient (its Gram matrix is singular to float64, so a plain
+    Cholesky QR fails); the shifted QR keeps block-COCG-rq going and every
+    column still matches its own single-orientation solve.  The single
+    solves use BiCGSTAB: with one circularly polarised right-hand side
+    b^T b = 0 (e0^T e0 = 0), so COCG at L = 1 breaks down (Q^T Q = 0) in the
+    reference as in this solver."""
+    say("nearly dependent orientations")
+    prob = list(_problem(L=4))
+    eu = np.array(prob[6], dtype=np.float64)
+    eu[1] = eu[0] + 1e-9
+    prob[6] = eu
+    dd = _gpu_dd(tuple(prob), "cocg_rq", 1e-9, 80)
+    assert dd.converge and dd.true_residual < 1e-8
+    for l in (0, 1, 3):
+        one = list(prob)
+        one[6] = eu[l:l + 1]
+        d1 = _gpu_dd(tuple(one), "bicgstab", 1e-10, 80)
+        assert rel(dd.X[:, l], d1.X[:, 0]) < 1e-7
+
+
 def test_nonconvergence_is_reported_not_raised():
     say("non-convergence")
     prob = _problem(L=3)

@@ -81,3 +81,21 @@ def test_surface_points_match_oracle():
     n2, g2 = o.cuboid_lattice(p2)
     assert np.array_equal(n1, n2) and np.array_equal(g1, g2)
     assert g.lattice_lf == o.lattice_lf and g.distance_factor == o.distance_factor
+
+
+def test_batch_entry_validates_arguments_without_a_gpu():
+    """bdda_solve_batch rejects an empty batch and a plan shared by two
+    instances before touching the device."""
+    import ctypes
+    from bdda_amd import _lib
+    lib = _lib.load()
+    vp = ctypes.c_void_p
+    nul = ctypes.POINTER(ctypes.c_int)()
+    nd = ctypes.POINTER(ctypes.c_double)()
+    rc = lib.bdda_solve_batch(0, None, 0, 0, None, None, None, None, 1e-5, 10, 20, 2, nul, nd, nd, nd, nul)
+    assert rc == 1 and b"nb >= 1" in lib.bdda_last_error()
+    fake = (vp * 2)(0x1000, 0x1000)
+    bufs = (vp * 2)(0x2000, 0x3000)
+    Ls = (ctypes.c_int * 2)(4, 4)
+    rc = lib.bdda_solve_batch(2, fake, 0, 0, bufs, None, bufs, Ls, 1e-5, 10, 20, 2, nul, nd, nd, nd, nul)
+    assert rc == 1 and b"own" in lib.bdda_last_error()
