@@ -108,8 +108,9 @@ def cpu_baseline(geometry, wl, gpu_iters, sample_L=16):
 
 def batch_leg(torch, nstreams=8):
     """Batched uneven instances (32 GRE particles, lattices 20..48 cells,
-    L = 16, mixed block-COCG-rq): one-at-a-time solves against one
-    concurrent batch on `nstreams` HIP streams, both after a warm-up."""
+    L = 16, block-COCG-rq to 1e-5): per precision, one-at-a-time solves
+    against one concurrent batch on `nstreams` HIP streams (both timed after
+    a warm-up)."""
     import numpy as np
     from bdda_amd import workloads
     from bdda_amd.batch import solve_batch
@@ -119,38 +120,41 @@ def batch_leg(torch, nstreams=8):
     geos = [workloads.build_geometry(w) for w in insts]
     t_geo = time.time() - t0
     dds = [workloads.make_dipoles(w, geometry=g, precision=wl["precision"]) for w, g in zip(insts, geos)]
-    for dd in dds:
-        dd.solve_matrix_equation()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for dd in dds:
-        dd.solve_matrix_equation()
-    torch.cuda.synchronize()
-    t_seq = time.perf_counter() - t0
-    X_seq = [dd.X_device.clone() for dd in dds]
-    solve_batch(dds, nstreams)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    solve_batch(dds, nstreams)
-    torch.cuda.synchronize()
-    t_bat = time.perf_counter() - t0
     n_or = sum(dd.L for dd in dds)
     N = [int(dd.num_element_occupy) for dd in dds]
-    its = [dd.iter_fin + 1 for dd in dds]
-    return {
-        "workload": wl["name"], "instances": len(dds), "value": round(n_or / t_bat, 2), "unit": "orientations/s",
-        "orientations": n_or, "batched_s": round(t_bat, 4), "sequential_s": round(t_seq, 4),
-        "sequential_value": round(n_or / t_seq, 2), "speedup_vs_sequential": round(t_seq / t_bat, 3),
-        "nstreams": nstreams, "precision": wl["precision"], "solver": wl["solver"],
-        "identical_to_sequential": all(torch.equal(dd.X_device, x) for dd, x in zip(dds, X_seq)),
-        "all_converged": all(dd.converge for dd in dds),
-        "max_true_residual": max(dd.true_residual for dd in dds),
-        "dipoles": {"min": min(N), "max": max(N), "total": sum(N)},
-        "longest_axis": {"min": int(min(max(dd.lattice_n) for dd in dds)),
-                         "max": int(max(max(dd.lattice_n) for dd in dds))},
-        "iterations": {"min": min(its), "max": max(its), "mean": round(float(np.mean(its)), 2)},
-        "geometry_s": round(t_geo, 1),
-    }
+    out = {"workload": wl["name"], "instances": len(dds), "unit": "orientations/s", "orientations": n_or,
+           "solver": wl["solver"], "nstreams": nstreams, "dipoles": {"min": min(N), "max": max(N), "total": sum(N)},
+           "longest_axis": {"min": int(min(max(dd.lattice_n) for dd in dds)),
+                            "max": int(max(max(dd.lattice_n) for dd in dds))},
+           "geometry_s": round(t_geo, 1)}
+    for prec in ("mixed", "fp64"):
+        for dd in dds:
+            dd.precision = prec
+            dd.solve_matrix_equation()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for dd in dds:
+            dd.solve_matrix_equation()
+        torch.cuda.synchronize()
+        t_seq = time.perf_counter() - t0
+        X_seq = [dd.X_device.clone() for dd in dds]
+        solve_batch(dds, nstreams)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        solve_batch(dds, nstreams)
+        torch.cuda.synchronize()
+        t_bat = time.perf_counter() - t0
+        its = [dd.iter_fin + 1 for dd in dds]
+        out[prec] = {
+            "value": round(n_or / t_bat, 2), "batched_s": round(t_bat, 4), "sequential_s": round(t_seq, 4),
+            "sequential_value": round(n_or / t_seq, 2), "speedup_vs_sequential": round(t_seq / t_bat, 3),
+            "identical_to_sequential": all(torch.equal(dd.X_device, x) for dd, x in zip(dds, X_seq)),
+            "all_converged": all(dd.converge for dd in dds),
+            "max_true_residual": max(dd.true_residual for dd in dds),
+            "iterations": {"min": min(its), "max": max(its), "mean": round(float(np.mean(its)), 2)},
+        }
+    out["value"] = out[wl["precision"]]["value"]
+    return out
 
 
 def main():
@@ -245,8 +249,9 @@ def main():
     if not a.no_batch:
         log("batched uneven instances")
         result["batch"] = batch_leg(torch)
-        log(f"batch: {result['batch']['value']} orientations/s, x{result['batch']['speedup_vs_sequential']} "
-            "vs one at a time")
+        b = result["batch"]
+        log(f"batch: mixed {b['mixed']['value']} / fp64 {b['fp64']['value']} orientations/s "
+            f"(x{b['mixed']['speedup_vs_sequential']} / x{b['fp64']['speedup_vs_sequential']} vs one at a time)")
 
     if not a.no_cpu_baseline:
         log("cpu baseline")

@@ -190,10 +190,13 @@ def train(model, data, steps=2000, batch=8, lr=2e-3, val_frac=0.05, seed=0, log=
     return dict(history=hist, val_loss=evaluate(model, data, val), val_index=val, train_time=time.time() - t0)
 
 
-def warm_start(model, dd):
+def warm_start(model, dd, project=True, chunk=64, residuals=None):
     """Initial guess X0 (3N, L) complex128 on the device for a set-up
     DiscreteDipoles, predicted on a cube of the model's grid (the lattice is
-    padded into it)."""
+    padded into it), `chunk` orientations per forward pass.  With `project`
+    the guess is replaced by its minimal-residual combination X0 C
+    (Plan.project_guess); `residuals`, if a list, receives the relative
+    residuals before and after."""
     lib = _lib.load()
     G = model.G
     cells, occ_h, origin = cube_geometry(dd, G)
@@ -203,17 +206,21 @@ def warm_start(model, dd):
     X0 = torch.zeros((3 * dd.num_element_occupy, L), dtype=torch.complex128, device="cuda")
     cells_d = torch.from_numpy(cells).cuda()
     alpha = torch.from_numpy(np.ascontiguousarray(dd.alpha_E[0], dtype=np.complex128)).cuda()
-    chunk = 8 if G <= 32 else 2
     for l0 in range(0, L, chunk):
         n = min(chunk, L - l0)
         inst = torch.zeros(n, dtype=torch.int32, device="cuda")
         x = build_inputs(lib, occ, inst, desc[l0:l0 + n].contiguous(), G, n)
         y = model.forward(x)
-        part = torch.zeros((3 * dd.num_element_occupy, n), dtype=torch.complex128, device="cuda")
+        part = X0 if n == L else torch.zeros((3 * dd.num_element_occupy, n), dtype=torch.complex128, device="cuda")
         check(lib.bdda_cube_exchange(_p(y), ctypes.c_void_p(part.data_ptr()), ctypes.c_void_p(cells_d.data_ptr()),
                                      ctypes.c_void_p(alpha.data_ptr()), len(cells), n, G, 0, _stream()),
               "cube_exchange")
-        X0[:, l0:l0 + n] = part
+        if part is not X0:
+            X0[:, l0:l0 + n] = part
+    if project:
+        r = dd._plan.project_guess(dd._B_dev, X0)
+        if residuals is not None:
+            residuals.extend(r)
     return X0
 
 

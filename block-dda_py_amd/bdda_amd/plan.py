@@ -136,6 +136,19 @@ class Plan:
         nh = flags.value >> 8
         return X, it.value, err.value, tr.value, hist[:nh].tolist(), bool(flags.value & 1)
 
+    def project_guess(self, B, X0):
+        """X0 (3N, L) complex128 device tensor, in place -> X0 C minimising
+        ||jpre (B - A X0 C)||_F; returns the relative residuals (before, after)."""
+        B = to_device(B)
+        if not (isinstance(X0, torch.Tensor) and X0.is_cuda and X0.dtype == torch.complex128
+                and X0.is_contiguous() and tuple(X0.shape) == tuple(B.shape)):
+            raise ValueError("X0 must be a contiguous complex128 device tensor shaped like B")
+        before = ctypes.c_double(0.0)
+        after = ctypes.c_double(0.0)
+        check(self.lib.bdda_project_guess(self.h, ptr(B), ptr(X0), B.shape[1], ctypes.byref(before),
+                                          ctypes.byref(after), _stream()), "project_guess")
+        return before.value, after.value
+
     def rhs(self, u, e0, k):
         u = np.ascontiguousarray(u, dtype=np.float64)
         e0 = np.ascontiguousarray(e0, dtype=np.complex128)

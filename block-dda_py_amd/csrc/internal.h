@@ -46,6 +46,8 @@ enum PassMode : int {
   PASS_CUBOID_Z = 4,    // cuboid (3n,L) -> z FFT -> T1
   PASS_CROP_Z = 5,      // T1 -> inverse z FFT -> cuboid (3n,L) (fft_mvp)
   PASS_PLAIN_INV = 6,   // PASS_PLAIN with the inverse transform (register kernels)
+  PASS_SCATTER_Z64 = 7, // PASS_SCATTER_Z reading a complex128 block (complex64 transform)
+  PASS_GATHER_Z64 = 8,  // PASS_GATHER_Z with a complex128 epilogue and output (complex64 transform)
 };
 
 struct LinePass {

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <atomic>
 #include <mutex>
+#include <type_traits>
 
 #include "internal.h"
 
@@ -30,6 +31,20 @@ __device__ __forceinline__ V cx(typename RealOf<V>::T x, typename RealOf<V>::T y
   v.y = y;
   return v;
 }
+template <typename D, typename S> __device__ __forceinline__ D cvt(S v) {
+  D d;
+  d.x = v.x;
+  d.y = v.y;
+  return d;
+}
+// Type of the occupied block at the scatter / gather boundary: the *_Z64
+// modes read and write complex128 around a complex64 transform.
+template <typename V, int MODE> struct BoundaryOf { using T = V; };
+template <typename V> struct BoundaryOf<V, PASS_SCATTER_Z64> { using T = double2; };
+template <typename V> struct BoundaryOf<V, PASS_GATHER_Z64> { using T = double2; };
+template <int MODE> constexpr bool is_scatter() { return MODE == PASS_SCATTER_Z || MODE == PASS_SCATTER_Z64; }
+template <int MODE> constexpr bool is_gather() { return MODE == PASS_GATHER_Z || MODE == PASS_GATHER_Z64; }
+
 // Guarded load returning a value: `c ? p[i] : zero` with two lvalues would
 // make the compiler load through a pointer select (a flat load that may hit a
 // private copy of zero), so the zero is produced as a value instead.
@@ -268,6 +283,7 @@ __device__ void green_multiply(V* buf, const LinePass& lp, int pt0, int np_here,
 
 template <typename V, int VPT, int MODE>
 __global__ void __launch_bounds__(512) k_line_pass(LinePass lp) {
+  using BV = typename BoundaryOf<V, MODE>::T;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int M = lp.m, P = lp.P, Q = lp.Q, QQ = P * Q;
   V* buf = reinterpret_cast<V*>(smem_raw);
@@ -289,7 +305,7 @@ __global__ void __launch_bounds__(512) k_line_pass(LinePass lp) {
   // ---- load: every global load of this thread (inputs, then twiddles) is
   // issued before any LDS store
   V v[VPT];
-  if (MODE == PASS_SCATTER_Z || MODE == PASS_CUBOID_Z) {
+  if (is_scatter<MODE>() || MODE == PASS_CUBOID_Z) {
     const int c = q / lp.Lb, l = lo * lp.Lb + q % lp.Lb;
     long long mrow[VPT];
 #pragma unroll
@@ -298,12 +314,12 @@ __global__ void __launch_bounds__(512) k_line_pass(LinePass lp) {
       mrow[i] = -1;
       if (active && e < lp.nin && l < lp.L) {
         const long long cell = (long long)pt * lp.nz + e;
-        mrow[i] = (MODE == PASS_SCATTER_Z) ? (long long)lp.inv[cell] : cell;
+        mrow[i] = is_scatter<MODE>() ? (long long)lp.inv[cell] : cell;
       }
     }
-    const V* __restrict__ S = reinterpret_cast<const V*>(lp.S);
+    const BV* __restrict__ S = reinterpret_cast<const BV*>(lp.S);
 #pragma unroll
-    for (int i = 0; i < VPT; ++i) v[i] = load_or_zero(mrow[i] >= 0, S + (3 * mrow[i] + c) * lp.L + l);
+    for (int i = 0; i < VPT; ++i) v[i] = cvt<V>(load_or_zero(mrow[i] >= 0, S + (3 * mrow[i] + c) * lp.L + l));
   } else {
     const int a = pt / lp.ptb, b = pt % lp.ptb;
     const V* __restrict__ base = src + a * lp.sa + b * lp.sb + q;
@@ -324,7 +340,7 @@ __global__ void __launch_bounds__(512) k_line_pass(LinePass lp) {
   __syncthreads();
 
   // ---- transform ------------------------------------------------------------
-  if (MODE == PASS_GATHER_Z || MODE == PASS_CROP_Z || (MODE == PASS_PLAIN && lp.inverse)) {
+  if (is_gather<MODE>() || MODE == PASS_CROP_Z || (MODE == PASS_PLAIN && lp.inverse)) {
     lds_fft<true, VPT>(buf, tws, lp, tm);
   } else {
     lds_fft<false, VPT>(buf, tws, lp, tm);
@@ -335,8 +351,9 @@ __global__ void __launch_bounds__(512) k_line_pass(LinePass lp) {
   }
 
   // ---- store (cropped to nout) ----------------------------------------------
-  if (MODE == PASS_GATHER_Z || MODE == PASS_CROP_Z) {
+  if (is_gather<MODE>() || MODE == PASS_CROP_Z) {
     using T = typename RealOf<V>::T;
+    using TB = typename RealOf<BV>::T;
     const T sc = (T)lp.scale;
     const int c = q / lp.Lb, l = lo * lp.Lb + q % lp.Lb;
     if (!active || l >= lp.L) return;
@@ -348,15 +365,17 @@ __global__ void __launch_bounds__(512) k_line_pass(LinePass lp) {
       row[i] = -1;
       if (e < lp.nout) {
         const long long cell = (long long)pt * lp.nz + e;
-        const long long m = (MODE == PASS_GATHER_Z) ? (long long)lp.inv[cell] : cell;
+        const long long m = is_gather<MODE>() ? (long long)lp.inv[cell] : cell;
         if (m >= 0) row[i] = 3 * m + c;
       }
     }
-    if (MODE == PASS_GATHER_Z) {
-      const V* __restrict__ S = reinterpret_cast<const V*>(lp.S);
-      const V* __restrict__ dg = reinterpret_cast<const V*>(lp.diag);
-      const V* __restrict__ jp = reinterpret_cast<const V*>(lp.jpre);
-      V sv[VPT], dv[VPT], jv[VPT];
+    if (is_gather<MODE>()) {
+      const BV* __restrict__ S = reinterpret_cast<const BV*>(lp.S);
+      const BV* __restrict__ dg = reinterpret_cast<const BV*>(lp.diag);
+      const BV* __restrict__ jp = reinterpret_cast<const BV*>(lp.jpre);
+      BV* __restrict__ outb = reinterpret_cast<BV*>(lp.AS);
+      const TB scb = (TB)lp.scale;
+      BV sv[VPT], dv[VPT], jv[VPT];
 #pragma unroll
       for (int i = 0; i < VPT; ++i) {
         if (row[i] >= 0) {
@@ -369,8 +388,8 @@ __global__ void __launch_bounds__(512) k_line_pass(LinePass lp) {
       for (int i = 0; i < VPT; ++i) {
         if (row[i] >= 0) {
           const int e = tm.rt + i * tm.TPC;
-          const V y = cscale(buf[e * QQ + tm.qq], sc);
-          out[row[i] * lp.L + l] = cmul(jv[i], cadd(cmul(dv[i], sv[i]), y));
+          const BV y = cscale(cvt<BV>(buf[e * QQ + tm.qq]), scb);
+          outb[row[i] * lp.L + l] = cmul(jv[i], cadd(cmul(dv[i], sv[i]), y));
         }
       }
     } else {
@@ -451,7 +470,8 @@ template <typename V, int R1, int R2, int MODE, int LB>
 __global__ void __launch_bounds__(512) k_reg2(LinePass lp) {
   constexpr int M = R1 * R2;
   constexpr int RMAX = R1 > R2 ? R1 : R2;
-  constexpr bool INV_PASS = (MODE == PASS_GATHER_Z || MODE == PASS_CROP_Z || MODE == PASS_PLAIN_INV);
+  constexpr bool INV_PASS = (is_gather<MODE>() || MODE == PASS_CROP_Z || MODE == PASS_PLAIN_INV);
+  using BV = typename BoundaryOf<V, MODE>::T;
   constexpr int PB = LB ? reg2_points(M, RMAX, 3 * LB) : 0;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int Lb = LB ? LB : lp.Lb;
@@ -488,7 +508,7 @@ __global__ void __launch_bounds__(512) k_reg2(LinePass lp) {
     // the twiddle table ----------------------------------------------------------
     V a[RMAX];
     const V* __restrict__ src = reinterpret_cast<const V*>(lp.src) + (long long)lo * lp.lo_src;
-    if (MODE == PASS_SCATTER_Z || MODE == PASS_CUBOID_Z) {
+    if (is_scatter<MODE>() || MODE == PASS_CUBOID_Z) {
       const int c = q / Lb, l = lo * Lb + q % Lb;
       long long mrow[R1];
 #pragma unroll
@@ -497,12 +517,13 @@ __global__ void __launch_bounds__(512) k_reg2(LinePass lp) {
         mrow[j1] = -1;
         if (s1 && active && e < lp.nin && l < lp.L) {
           const long long cell = (long long)pt * lp.nz + e;
-          mrow[j1] = (MODE == PASS_SCATTER_Z) ? (long long)lp.inv[cell] : cell;
+          mrow[j1] = is_scatter<MODE>() ? (long long)lp.inv[cell] : cell;
         }
       }
-      const V* __restrict__ S = reinterpret_cast<const V*>(lp.S);
+      const BV* __restrict__ S = reinterpret_cast<const BV*>(lp.S);
 #pragma unroll
-      for (int j1 = 0; j1 < R1; ++j1) a[j1] = load_or_zero(mrow[j1] >= 0, S + (3 * mrow[j1] + c) * lp.L + l);
+      for (int j1 = 0; j1 < R1; ++j1)
+        a[j1] = cvt<V>(load_or_zero(mrow[j1] >= 0, S + (3 * mrow[j1] + c) * lp.L + l));
     } else {
       const int ga = pt / lp.ptb, gb = pt % lp.ptb;
       const int se = (int)lp.se;  // a line spans fewer than 2^31 elements (checked at launch)
@@ -523,8 +544,8 @@ __global__ void __launch_bounds__(512) k_reg2(LinePass lp) {
     }
     // z-inverse with gather: occupied index of each output cell this thread
     // stores (-1 for empty cells and cropped elements)
-    int outm[MODE == PASS_GATHER_Z ? R2 : 1];
-    if (MODE == PASS_GATHER_Z) {
+    int outm[is_gather<MODE>() ? R2 : 1];
+    if (is_gather<MODE>()) {
 #pragma unroll
       for (int k2 = 0; k2 < R2; ++k2) {
         const int e = k1 + R1 * k2;
@@ -591,8 +612,9 @@ __global__ void __launch_bounds__(512) k_reg2(LinePass lp) {
     }
     // ---- store X[k1 + R1 k2], k < nout ---------------------------------------------
     if (s2 && active) {
-      if (MODE == PASS_GATHER_Z || MODE == PASS_CROP_Z) {
+      if (is_gather<MODE>() || MODE == PASS_CROP_Z) {
         using T = typename RealOf<V>::T;
+        using TB = typename RealOf<BV>::T;
         const T sc = (T)lp.scale;
         const int c = q / Lb, l = lo * Lb + q % Lb;
         if (l < lp.L) {
@@ -602,18 +624,20 @@ __global__ void __launch_bounds__(512) k_reg2(LinePass lp) {
 #pragma unroll
           for (int k2 = 0; k2 < R2; ++k2) {
             const int e = k1 + R1 * k2;
-            const int m = (MODE == PASS_GATHER_Z) ? outm[k2] : (e < lp.nout ? pt * lp.nz + e : -1);
+            const int m = is_gather<MODE>() ? outm[k2] : (e < lp.nout ? pt * lp.nz + e : -1);
             row[k2] = m >= 0 ? 3 * m + c : -1;
           }
-          if (MODE == PASS_GATHER_Z) {
-            const V* __restrict__ S = reinterpret_cast<const V*>(lp.S);
-            const V* __restrict__ dg = reinterpret_cast<const V*>(lp.diag);
-            const V* __restrict__ jp = reinterpret_cast<const V*>(lp.jpre);
+          if (is_gather<MODE>()) {
+            const BV* __restrict__ S = reinterpret_cast<const BV*>(lp.S);
+            const BV* __restrict__ dg = reinterpret_cast<const BV*>(lp.diag);
+            const BV* __restrict__ jp = reinterpret_cast<const BV*>(lp.jpre);
+            BV* __restrict__ outb = reinterpret_cast<BV*>(lp.AS);
+            const TB scb = (TB)lp.scale;
             // groups of four outputs keep the staged operands within budget
             constexpr int H = R2 < 4 ? R2 : 4;
 #pragma unroll
             for (int k0 = 0; k0 < R2; k0 += H) {
-              V sv[H], dv[H], jv[H];
+              BV sv[H], dv[H], jv[H];
 #pragma unroll
               for (int h = 0; h < H && k0 + h < R2; ++h) {
                 const int r = row[k0 + h];
@@ -624,7 +648,9 @@ __global__ void __launch_bounds__(512) k_reg2(LinePass lp) {
 #pragma unroll
               for (int h = 0; h < H && k0 + h < R2; ++h) {
                 const int r = row[k0 + h];
-                if (r >= 0) out[(long long)r * lp.L + l] = cmul(jv[h], cadd(cmul(dv[h], sv[h]), cscale(a[k0 + h], sc)));
+                if (r >= 0)
+                  outb[(long long)r * lp.L + l] =
+                      cmul(jv[h], cadd(cmul(dv[h], sv[h]), cscale(cvt<BV>(a[k0 + h]), scb)));
               }
             }
           } else {
@@ -681,6 +707,17 @@ static void launch_reg2_tile(const LinePass& lp, dim3 grid, int nthr, size_t lds
     case PASS_SCATTER_Z: launch_reg2_mode<V, R1, R2, PASS_SCATTER_Z, LB>(lp, grid, nthr, lds, s); break;
     case PASS_CONV_X: launch_reg2_mode<V, R1, R2, PASS_CONV_X, LB>(lp, grid, nthr, lds, s); break;
     case PASS_GATHER_Z: launch_reg2_mode<V, R1, R2, PASS_GATHER_Z, LB>(lp, grid, nthr, lds, s); break;
+    case PASS_SCATTER_Z64:
+    case PASS_GATHER_Z64:
+      if constexpr (std::is_same<V, float2>::value) {
+        if (mode == PASS_SCATTER_Z64)
+          launch_reg2_mode<V, R1, R2, PASS_SCATTER_Z64, LB>(lp, grid, nthr, lds, s);
+        else
+          launch_reg2_mode<V, R1, R2, PASS_GATHER_Z64, LB>(lp, grid, nthr, lds, s);
+      } else {
+        set_error("register FFT: complex128 boundary modes need a complex64 transform");
+      }
+      break;
     case PASS_CUBOID_Z: launch_reg2_mode<V, R1, R2, PASS_CUBOID_Z, LB>(lp, grid, nthr, lds, s); break;
     default: launch_reg2_mode<V, R1, R2, PASS_CROP_Z, LB>(lp, grid, nthr, lds, s); break;
   }
@@ -781,6 +818,17 @@ void launch_line_pass(const LinePass& lp0, hipStream_t s) {
     case PASS_CONV_X: launch_mode<V, PASS_CONV_X>(lp, grid, nthr, lds, s); break;
     case PASS_GATHER_Z: launch_mode<V, PASS_GATHER_Z>(lp, grid, nthr, lds, s); break;
     case PASS_CUBOID_Z: launch_mode<V, PASS_CUBOID_Z>(lp, grid, nthr, lds, s); break;
+    case PASS_SCATTER_Z64:
+    case PASS_GATHER_Z64:
+      if constexpr (std::is_same<V, float2>::value) {
+        if (lp.mode == PASS_SCATTER_Z64)
+          launch_mode<V, PASS_SCATTER_Z64>(lp, grid, nthr, lds, s);
+        else
+          launch_mode<V, PASS_GATHER_Z64>(lp, grid, nthr, lds, s);
+      } else {
+        set_error("line FFT: complex128 boundary modes need a complex64 transform");
+      }
+      break;
     default: launch_mode<V, PASS_CROP_Z>(lp, grid, nthr, lds, s); break;
   }
 }

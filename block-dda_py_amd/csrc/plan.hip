@@ -14,6 +14,7 @@
 #include <memory>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/bdda.h"
@@ -98,12 +99,11 @@ struct bdda_plan {
   // Green half spectrum
   bdda::DevBuf g64, g32;
   // target
-  bdda::DevBuf inv, pos, diag64, jpre64, diag32, jpre32;
+  bdda::DevBuf inv, pos, diag64, jpre64;
   // FFT intermediates
   bdda::DevBuf T1, T2;
   // solver workspace (blocks of 3N x L)
   bdda::DevBuf blk[12];
-  bdda::DevBuf blk32[10];
   bdda::DevBuf gram_part, gram_out, coef, dots_d, dots_n, obs_part, Bj64;
   bdda::HostBuf h_gram, h_coef, h_dots;
   // optional per-pass kernel timing (HIP events recorded around each pass)
@@ -137,8 +137,9 @@ template <> struct PrecTraits<double2> {
 template <> struct PrecTraits<float2> {
   static const void* tw(const bdda_plan* p, int a) { return p->tw32[a].p; }
   static const void* green(const bdda_plan* p) { return p->g32.p; }
-  static const void* diag(const bdda_plan* p) { return p->diag32.p; }
-  static const void* jpre(const bdda_plan* p) { return p->jpre32.p; }
+  // occupied blocks are always complex128 (see apply_operator)
+  static const void* diag(const bdda_plan*) { return nullptr; }
+  static const void* jpre(const bdda_plan*) { return nullptr; }
 };
 
 // Column tile Lb: every line-FFT tile (m x 3 Lb complex) must fit 4096
@@ -200,8 +201,16 @@ static int ensure_fft_ws(bdda_plan* p, int L) {
 // (forward, Green multiply, inverse), inverse y pass, inverse z pass (gather).
 // occupied=true: S/AS are (3N, L) occupied blocks and the epilogue applies
 // jpre * (diag * S + .); otherwise (3n, L) cuboid blocks and AS = A_off S.
-template <typename V>
-static int apply_operator(bdda_plan* p, const V* S, V* AS, int L, bool occupied, hipStream_t s) {
+// V is the transform's type; BV (occupied blocks only) the block's: a
+// complex64 transform of complex128 blocks reads S and writes AS in
+// complex128 (the epilogue in float64).
+template <typename V, typename BV = V>
+static int apply_operator(bdda_plan* p, const BV* S, BV* AS, int L, bool occupied, hipStream_t s) {
+  constexpr bool wide = sizeof(BV) > sizeof(V);
+  if ((occupied && sizeof(BV) != sizeof(double2)) || (wide && !occupied)) {
+    set_error("occupied blocks are complex128 (cuboid blocks take the transform's type)");
+    return BDDA_E_ARG;
+  }
   const int Lb = choose_lb(p, L);
   const int Q = 3 * Lb;
   const int nlo = (L + Lb - 1) / Lb;
@@ -223,8 +232,8 @@ static int apply_operator(bdda_plan* p, const V* S, V* AS, int L, bool occupied,
   base.inv = p->inv.as<int>();
   base.S = S;
   base.AS = AS;
-  base.diag = PrecTraits<V>::diag(p);
-  base.jpre = PrecTraits<V>::jpre(p);
+  base.diag = PrecTraits<BV>::diag(p);
+  base.jpre = PrecTraits<BV>::jpre(p);
   base.mx = mx;
   base.my = my;
   base.mz = mz;
@@ -242,7 +251,7 @@ static int apply_operator(bdda_plan* p, const V* S, V* AS, int L, bool occupied,
   {
     LinePass lp = base;
     axis(lp, 2);
-    lp.mode = occupied ? PASS_SCATTER_Z : PASS_CUBOID_Z;
+    lp.mode = occupied ? (wide ? PASS_SCATTER_Z64 : PASS_SCATTER_Z) : PASS_CUBOID_Z;
     lp.nin = nz;
     lp.nout = mz;
     lp.npts = nx * ny;
@@ -327,7 +336,7 @@ static int apply_operator(bdda_plan* p, const V* S, V* AS, int L, bool occupied,
   {
     LinePass lp = base;
     axis(lp, 2);
-    lp.mode = occupied ? PASS_GATHER_Z : PASS_CROP_Z;
+    lp.mode = occupied ? (wide ? PASS_GATHER_Z64 : PASS_GATHER_Z) : PASS_CROP_Z;
     lp.inverse = 1;
     lp.nin = mz;
     lp.nout = nz;
@@ -352,19 +361,26 @@ struct Ctx {
   long long rows;
   hipStream_t s;
   std::vector<double>* hist;
-  // inner complex64 solves of the mixed mode end early once they stall, so
-  // that the float64 refinement restarts them from the true residual
+  // inner solves of the mixed mode end early once they stall, or when their
+  // block QR breaks down, so that the float64 refinement restarts them from
+  // the true residual
   bool stop_on_stall = false;
+  // mixed mode: the operator's convolution runs in complex64 (complex128
+  // blocks at its scatter / gather boundaries)
+  bool conv32 = false;
 };
 
-// The last two residuals both failed to improve on the best one before them.
+// No real progress over the last kStallWindow iterations: their best
+// residual is not 10% below the best one before them (block COCG residuals
+// rise and fall, so a shorter window would cut healthy iterations short).
+constexpr int kStallWindow = 6;
 static bool stalled(const Ctx& c) {
-  if (!c.stop_on_stall || !c.hist || c.hist->size() < 3) return false;
+  if (!c.stop_on_stall || !c.hist || (int)c.hist->size() <= kStallWindow + 1) return false;
   const std::vector<double>& h = *c.hist;
-  const size_t k = h.size() - 1;
-  double best = h[0];
-  for (size_t i = 1; i + 1 < k; ++i) best = std::min(best, h[i]);
-  return h[k - 1] >= best && h[k] >= best;
+  const size_t split = h.size() - kStallWindow;
+  const double before = *std::min_element(h.begin(), h.begin() + split);
+  const double recent = *std::min_element(h.begin() + split, h.end());
+  return recent > 0.9 * before;
 }
 
 static int gram_to_host(Ctx& c, const void* A, const void* B, bool conj, bool f32, std::vector<zc>& out) {
@@ -470,6 +486,9 @@ template <typename V> static constexpr bool is_f32() { return sizeof(V) == sizeo
 
 template <typename V>
 static int op(Ctx& c, const V* S, V* AS) {
+  if constexpr (std::is_same<V, double2>::value) {
+    if (c.conv32) return apply_operator<float2, double2>(c.p, S, AS, c.L, true, c.s);
+  }
   return apply_operator<V>(c.p, S, AS, c.L, true, c.s);
 }
 
@@ -547,40 +566,45 @@ static int solve_bicgstab(Ctx& c, const V* Bj, V* X, bool x_zero, double bnorm, 
 template <typename V>
 static int cholqr2(Ctx& c, const V* W, V* Q1, V* Qn, std::vector<zc>& tau) {
   const int L = c.L;
-  std::vector<zc> G, R1, R1i, R2, R2i;
-  int rc;
-  if ((rc = gram_to_host(c, W, W, true, is_f32<V>(), G))) return rc;
-  bool shifted = false;
-  if (!la::cholesky_upper(G, R1, L)) {
-    // Nearly rank-deficient block (the Gram squares its condition number):
-    // shifted CholeskyQR3 -- factor W^H W + s I with
-    // s = 11 (m n + n (n + 1)) u tr(W^H W), which always succeeds for
-    // cond(W) below 1/u, then two unshifted passes restore orthogonality.
-    double tr = 0.0;
-    for (int i = 0; i < L; ++i) tr += G[(size_t)i * L + i].real();
-    const double u = std::ldexp(1.0, -53);
-    const double shift = 11.0 * ((double)c.rows * L + (double)L * (L + 1)) * u * tr;
-    for (int i = 0; i < L; ++i) G[(size_t)i * L + i] += shift;
-    if (!(tr > 0.0) || !la::cholesky_upper(G, R1, L)) return singular("block QR (Cholesky of W^H W)");
-    shifted = true;
+  std::vector<zc> G, Ri, R1, R2, R3, t;
+  int rc = 0;
+  // one Cholesky QR pass: out = in R^-1 with R^H R = in^H in + shift I
+  // (*ok = false when the Cholesky factorisation breaks down)
+  auto pass = [&](const V* in, V* out, std::vector<zc>& R, bool shifted, int slot, bool* ok) -> int {
+    int e = gram_to_host(c, in, in, true, is_f32<V>(), G);
+    if (e) return e;
+    if (shifted) {
+      // shifted CholeskyQR: s = 11 (m n + n (n + 1)) u tr(G) makes G + s I
+      // positive definite for any block with cond < 1/u
+      double tr = 0.0;
+      for (int i = 0; i < L; ++i) tr += G[(size_t)i * L + i].real();
+      const double shift = 11.0 * ((double)c.rows * L + (double)L * (L + 1)) * std::ldexp(1.0, -53) * tr;
+      for (int i = 0; i < L; ++i) G[(size_t)i * L + i] += shift;
+    }
+    *ok = la::cholesky_upper(G, R, L);
+    if (!*ok) return 0;
+    la::upper_inverse(R, Ri, L);
+    return update<V>(c, out, nullptr, 0.0, in, nullptr, 0.0, Ri, slot, nullptr, 0.0);
+  };
+  bool ok1 = false, ok2 = false, ok3 = false;
+  if ((rc = pass(W, Q1, R1, false, 3, &ok1))) return rc;
+  if (ok1 && (rc = pass(Q1, Qn, R2, false, 4, &ok2))) return rc;
+  if (ok1 && ok2) {
+    la::matmul(R2, false, R1, tau, L);
+    return 0;
   }
-  la::upper_inverse(R1, R1i, L);
-  if ((rc = update<V>(c, Q1, nullptr, 0.0, W, nullptr, 0.0, R1i, 3, nullptr, 0.0))) return rc;
-  if ((rc = gram_to_host(c, Q1, Q1, true, is_f32<V>(), G))) return rc;
-  if (!la::cholesky_upper(G, R2, L)) return singular("block QR (Cholesky of Q1^H Q1)");
-  la::upper_inverse(R2, R2i, L);
-  if ((rc = update<V>(c, Qn, nullptr, 0.0, Q1, nullptr, 0.0, R2i, 4, nullptr, 0.0))) return rc;
-  la::matmul(R2, false, R1, tau, L);
-  if (shifted) {
-    std::vector<zc> R3, R3i, t2;
-    if ((rc = gram_to_host(c, Qn, Qn, true, is_f32<V>(), G))) return rc;
-    if (!la::cholesky_upper(G, R3, L)) return singular("block QR (Cholesky of Q2^H Q2)");
-    la::upper_inverse(R3, R3i, L);
-    if ((rc = update<V>(c, Q1, nullptr, 0.0, Qn, nullptr, 0.0, R3i, 4, nullptr, 0.0))) return rc;
-    BDDA_HIP(hipMemcpyAsync(Qn, Q1, (size_t)c.rows * L * sizeof(V), hipMemcpyDeviceToDevice, c.s));
-    la::matmul(R3, false, tau, t2, L);
-    tau.swap(t2);
-  }
+  // Nearly rank-deficient block (the Gram matrix squares its condition
+  // number): shifted CholeskyQR3 from W -- a shifted pass, then two plain
+  // passes that restore orthogonality.
+  if ((rc = pass(W, Q1, R1, true, 3, &ok1))) return rc;
+  if (!ok1) return singular("block QR (Cholesky of W^H W)");
+  if ((rc = pass(Q1, Qn, R2, false, 4, &ok2))) return rc;
+  if (!ok2) return singular("block QR (Cholesky of Q1^H Q1)");
+  if ((rc = pass(Qn, Q1, R3, false, 4, &ok3))) return rc;
+  if (!ok3) return singular("block QR (Cholesky of Q2^H Q2)");
+  BDDA_HIP(hipMemcpyAsync(Qn, Q1, (size_t)c.rows * L * sizeof(V), hipMemcpyDeviceToDevice, c.s));
+  la::matmul(R2, false, R1, t, L);
+  la::matmul(R3, false, t, tau, L);
   return 0;
 }
 
@@ -896,8 +920,6 @@ int bdda_set_target(bdda_plan* p, const int64_t* address, int64_t N, const doubl
   if (!rc) rc = p->pos.ensure(N * 3 * sizeof(double));
   if (!rc) rc = p->diag64.ensure(3 * N * sizeof(double2));
   if (!rc) rc = p->jpre64.ensure(3 * N * sizeof(double2));
-  if (!rc) rc = p->diag32.ensure(3 * N * sizeof(float2));
-  if (!rc) rc = p->jpre32.ensure(3 * N * sizeof(float2));
   if (rc) return rc;
   DevBuf addr;
   if ((rc = addr.ensure(N * sizeof(long long)))) return rc;
@@ -907,8 +929,6 @@ int bdda_set_target(bdda_plan* p, const int64_t* address, int64_t N, const doubl
   p->has_pos = pos != nullptr;
   BDDA_HIP(hipMemcpy(p->diag64.p, diag, 3 * N * sizeof(double2), hipMemcpyHostToDevice));
   BDDA_HIP(hipMemcpy(p->jpre64.p, jpre, 3 * N * sizeof(double2), hipMemcpyHostToDevice));
-  launch_convert<float2, double2>(p->diag32.as<float2>(), p->diag64.as<double2>(), 3 * N, 1.0, 0);
-  launch_convert<float2, double2>(p->jpre32.as<float2>(), p->jpre64.as<double2>(), 3 * N, 1.0, 0);
   BDDA_HIP(hipGetLastError());
   BDDA_HIP(hipDeviceSynchronize());
   p->N = N;
@@ -983,15 +1003,23 @@ int bdda_solve(bdda_plan* p, int solver, int precision, const void* B_dev, const
     for (int i = 0; i < 7; ++i) ws[i] = p->blk[i].as<double2>();
     if ((rc = run_solver<double2>(c, solver, Bj, X, x_zero, bnorm, tol, itermax, ws, &it, &err))) return rc;
   } else {
-    // mixed precision: complex64 inner solves on the current true residual,
-    // float64 accumulation of X and a float64 true residual after each cycle
-    for (int i = 0; i < 9 && !rc; ++i) rc = p->blk32[i].ensure(b32);
-    if (rc) return rc;
-    float2* ws[7];
-    for (int i = 0; i < 7; ++i) ws[i] = p->blk32[i].as<float2>();
-    float2* R32 = p->blk32[7].as<float2>();
-    float2* D32 = p->blk32[8].as<float2>();
+    // mixed precision: the Goodman convolution (nearly all of the operator's
+    // bytes) runs in complex64 between complex128 scatter / gather
+    // boundaries; the Krylov blocks, their updates, Gram matrices and L x L
+    // solves stay float64.  Cycles of at most refine_every iterations solve
+    // for the correction D on the float64 true residual R = Bj - A X, then
+    // X += D and R is recomputed with the float64 operator.
+    // A cycle that does not halve the true residual (the Krylov recurrence
+    // amplifying the complex64 rounding on a hard problem) hands the
+    // remaining cycles to the complex128 convolution; one that makes the
+    // residual worse is undone first.
+    if ((rc = p->blk[9].ensure(b64)) || (rc = p->blk[10].ensure(b64))) return rc;
+    double2* ws[7];
+    for (int i = 0; i < 7; ++i) ws[i] = p->blk[i].as<double2>();
     double2* R64 = p->blk[7].as<double2>();
+    double2* D = p->blk[9].as<double2>();
+    double2* Rsave = p->blk[10].as<double2>();
+    bool conv32 = true;
     if (x_zero) {
       BDDA_HIP(hipMemsetAsync(X, 0, b64, s));
       BDDA_HIP(hipMemcpyAsync(R64, Bj, b64, hipMemcpyDeviceToDevice, s));
@@ -1008,25 +1036,50 @@ int bdda_solve(bdda_plan* p, int solver, int precision, const void* B_dev, const
     err = res;
     while (res >= tol && total < itermax) {
       const double rs = std::sqrt(r2);
-      launch_convert<float2, double2>(R32, R64, n, 1.0 / rs, s);
-      BDDA_HIP(hipGetLastError());
+      // the complex64 convolution bounds what a cycle can reach
       double inner_tol = 0.5 * tol * bnorm / rs;
       if (inner_tol < 2e-6) inner_tol = 2e-6;
       std::vector<double> inner;
       c.hist = &inner;
       c.stop_on_stall = true;
+      c.conv32 = conv32;
       int iit = 0;
       double ierr = 0.0;
       const int budget = std::min(cycle, itermax - total);
-      if ((rc = run_solver<float2>(c, solver, R32, D32, true, 1.0, inner_tol, budget, ws, &iit, &ierr))) return rc;
-      c.hist = &hist;
+      rc = run_solver<double2>(c, solver, R64, D, true, rs, inner_tol, budget, ws, &iit, &ierr);
+      c.conv32 = false;
       c.stop_on_stall = false;
+      c.hist = &hist;
+      // a singular block QR / L x L system ends the cycle; the solvers update
+      // D in place, so the correction reached so far stands
+      if (rc == BDDA_E_SINGULAR) {
+        g_err.clear();
+        rc = 0;
+        if (inner.empty()) {
+          if (!conv32) break;
+          conv32 = false;
+          continue;
+        }
+      }
+      if (rc) return rc;
       for (double e : inner) hist.push_back(e * rs / bnorm);
       total += (int)inner.size();
-      launch_add_c64_to_c128(X, D32, n, rs, s);
+      if (conv32) BDDA_HIP(hipMemcpyAsync(Rsave, R64, b64, hipMemcpyDeviceToDevice, s));
+      launch_axpy_norm<double2>(X, X, make_double2(1.0, 0.0), D, n, nullptr, dot_parts(), s);
       BDDA_HIP(hipGetLastError());
       if ((rc = apply_operator<double2>(p, X, tmp, L, true, s))) return rc;
-      if ((rc = axpy_norm_host<double2>(c, R64, Bj, zc(-1.0), tmp, &r2))) return rc;
+      double r2new;
+      if ((rc = axpy_norm_host<double2>(c, R64, Bj, zc(-1.0), tmp, &r2new))) return rc;
+      if (conv32 && r2new > 0.25 * r2) {
+        conv32 = false;
+        if (r2new >= r2) {  // undo the cycle
+          launch_axpy_norm<double2>(X, X, make_double2(-1.0, 0.0), D, n, nullptr, dot_parts(), s);
+          BDDA_HIP(hipGetLastError());
+          BDDA_HIP(hipMemcpyAsync(R64, Rsave, b64, hipMemcpyDeviceToDevice, s));
+          r2new = r2;
+        }
+      }
+      r2 = r2new;
       res = std::sqrt(r2) / bnorm;
       if (!hist.empty()) hist.back() = res;
       if (inner.empty()) break;
@@ -1111,6 +1164,60 @@ int bdda_solve_batch(int nb, bdda_plan* const* plans, int solver, int precision,
       return rcs[i];
     }
   }
+  return 0;
+}
+
+int bdda_project_guess(bdda_plan* p, const void* B_dev, void* X0_dev, int L, double* res_before,
+                       double* res_after, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  g_err.clear();
+  if (!p->green_ready || !p->target_ready) {
+    set_error("bdda_project_guess: call bdda_green_init and bdda_set_target first");
+    return BDDA_E_STATE;
+  }
+  if (L < 1 || L > 128 || !B_dev || !X0_dev) {
+    set_error("bdda_project_guess: bad arguments (1 <= L <= 128, B and X0 on the device)");
+    return BDDA_E_ARG;
+  }
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  Ctx c{p, L, 3 * p->N, s, nullptr};
+  const size_t b64 = (size_t)c.rows * L * sizeof(double2);
+  int rc = p->Bj64.ensure(b64);
+  for (int i = 0; i < 3 && !rc; ++i) rc = p->blk[i].ensure(b64);
+  if (rc) return rc;
+  double2* X0 = (double2*)X0_dev;
+  double2* Bj = p->Bj64.as<double2>();
+  double2* W = p->blk[0].as<double2>();
+  double2* Y = p->blk[1].as<double2>();
+  double2* R = p->blk[2].as<double2>();
+  launch_scale_rows<double2>(Bj, (const double2*)B_dev, p->jpre64.as<double2>(), c.rows, L, s);
+  BDDA_HIP(hipGetLastError());
+  if ((rc = apply_operator<double2>(p, X0, W, L, true, s))) return rc;  // W = jpre (A X0)
+  double bn2 = 0.0, r0 = 0.0, r1 = 0.0;
+  if ((rc = frob2(c, Bj, &bn2))) return rc;
+  if ((rc = axpy_norm_host<double2>(c, R, Bj, zc(-1.0), W, &r0))) return rc;
+  // C = argmin ||Bj - W C||_F : (W^H W) C = W^H Bj, with a relative ridge
+  // of 1e-13 tr(W^H W) / L against nearly dependent guesses
+  std::vector<zc> G, h;
+  std::vector<int> piv;
+  if ((rc = gram_to_host(c, W, W, true, false, G))) return rc;
+  if ((rc = gram_to_host(c, W, Bj, true, false, h))) return rc;
+  double tr = 0.0;
+  for (int i = 0; i < L; ++i) tr += G[(size_t)i * L + i].real();
+  for (int i = 0; i < L; ++i) G[(size_t)i * L + i] += 1e-13 * tr / L;
+  if (!(tr > 0.0) || !la::lu_factor(G, piv, L)) return singular("guess projection (W^H W)");
+  la::lu_solve(G, piv, L, h, L);
+  // X0 <- X0 C ; residual Bj - W C
+  if ((rc = update<double2>(c, Y, nullptr, 0.0, X0, nullptr, 0.0, h, 0, nullptr, 0.0))) return rc;
+  BDDA_HIP(hipMemcpyAsync(X0, Y, b64, hipMemcpyDeviceToDevice, s));
+  std::vector<zc> negC(h);
+  for (auto& v : negC) v = -v;
+  if ((rc = update<double2>(c, R, Bj, 1.0, W, nullptr, 0.0, negC, 1, nullptr, 0.0))) return rc;
+  if ((rc = frob2(c, R, &r1))) return rc;
+  BDDA_HIP(hipStreamSynchronize(s));
+  const double bn = std::sqrt(bn2);
+  if (res_before) *res_before = bn > 0 ? std::sqrt(r0) / bn : 0.0;
+  if (res_after) *res_after = bn > 0 ? std::sqrt(r1) / bn : 0.0;
   return 0;
 }
 

@@ -8,8 +8,9 @@
 2. trains the 4-layer, width-32, 12-mode Fourier neural operator with its
    own forward/backward kernels (bf16 MFMA, float32 accumulation) and Adam;
 3. compares mixed-precision block-COCG-rq on held-out isotropic particles
-   (L = 64) from a zero start and from the network's guess: iterations and
-   time to relative residual 1e-5, warm-start time included.
+   (L = 64) from a zero start, from the network's raw guess and from its
+   minimal-residual projection: iterations and time to relative residual
+   1e-5, inference and projection time included.
 Prints one JSON summary line.
 """
 import argparse
@@ -53,13 +54,20 @@ def gain(model, n_eval=4, L=64, G=32, seed=12345):
         torch.cuda.synchronize()
         t_zero, it_zero, res_zero = time.perf_counter() - t0, dd.iter_fin + 1, dd.true_residual
         t0 = time.perf_counter()
-        dd.X0 = warm_start(model, dd)
+        dd.X0 = warm_start(model, dd, project=False)
+        dd.solve_matrix_equation()
+        torch.cuda.synchronize()
+        t_raw, it_raw = time.perf_counter() - t0, dd.iter_fin + 1
+        rr = []
+        t0 = time.perf_counter()
+        dd.X0 = warm_start(model, dd, residuals=rr)
         dd.solve_matrix_equation()
         torch.cuda.synchronize()
         t_warm, it_warm, res_warm = time.perf_counter() - t0, dd.iter_fin + 1, dd.true_residual
         rows.append(dict(N=int(dd.num_element_occupy), lattice=[int(v) for v in dd.lattice_n],
-                         it_zero=it_zero, it_warm=it_warm, t_zero_ms=round(t_zero * 1e3, 2),
-                         t_warm_ms=round(t_warm * 1e3, 2), res_zero=res_zero, res_warm=res_warm,
+                         it_zero=it_zero, it_raw_guess=it_raw, it_warm=it_warm, t_zero_ms=round(t_zero * 1e3, 2),
+                         t_raw_guess_ms=round(t_raw * 1e3, 2), t_warm_ms=round(t_warm * 1e3, 2),
+                         guess_residual=rr[0], projected_residual=rr[1], res_zero=res_zero, res_warm=res_warm,
                          warm_used=bool(dd.warm_start_used)))
         log(f"eval {i}: {rows[-1]}")
     return rows

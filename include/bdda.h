@@ -93,9 +93,12 @@ int bdda_block_mvp(bdda_plan* plan, const void* S_dev, void* AS_dev, int L, void
 /* Block-Krylov solve of A X = B with left Jacobi preconditioning.
  *   solver        BDDA_SOLVER_COCG_RQ | BDDA_SOLVER_BICGSTAB
  *   precision     BDDA_PREC_FP64: the reference algorithm in complex128;
- *                 BDDA_PREC_MIXED: complex64 operator and vector updates,
- *                 float64 L x L solves / block QR, true float64 residual and
- *                 refinement every refine_every iterations.
+ *                 BDDA_PREC_MIXED: the operator's Goodman convolution in
+ *                 complex64 (complex128 scatter / gather boundaries and
+ *                 Jacobi epilogue); Krylov blocks, updates, block QR and
+ *                 L x L solves in float64; cycles of at most refine_every
+ *                 iterations on the float64 true residual, which is
+ *                 recomputed with the complex128 operator after each cycle.
  *   B_dev         (3N, L) complex128 right-hand side
  *   X0_dev        (3N, L) complex128 initial guess or NULL (zero start); a
  *                 guess whose preconditioned residual exceeds ||jpre B|| is
@@ -113,6 +116,17 @@ int bdda_block_mvp(bdda_plan* plan, const void* S_dev, void* AS_dev, int L, void
 int bdda_solve(bdda_plan* plan, int solver, int precision, const void* B_dev, const void* X0_dev, void* X_dev,
                int L, double tol, int itermax, int refine_every, int* iter_fin, double* err_fin,
                double* true_res, double* err_hist, int* flags_out, void* stream);
+
+/* Minimal-residual projection of an initial guess: X0_dev (3N, L)
+ * complex128 is replaced by X0 C, where the L x L matrix C minimises
+ * || jpre (B - A X0 C) ||_F over the span of the L guessed columns (one
+ * complex128 operator application, two Gram matrices, an L x L solve).  The
+ * result is never worse than the guess itself (C = I is admissible) nor than
+ * zero (C = 0).  res_before / res_after (nullable): preconditioned relative
+ * residual of the guess before and after.  No counterpart in the reference
+ * (its solvers start from zero); meant for learned warm starts. */
+int bdda_project_guess(bdda_plan* plan, const void* B_dev, void* X0_dev, int L, double* res_before,
+                       double* res_after, void* stream);
 
 /* Batched uneven instances: solves nb independent problems, instance i on
  * plans[i] (its own plan, with its own FFT lengths, column tile and Green
