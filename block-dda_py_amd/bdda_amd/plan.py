@@ -68,6 +68,13 @@ class Plan:
         check(self.lib.bdda_green_download(self.h, ptr(out)), "green_download")
         return out
 
+    def column_tile(self, L):
+        """Orientations per line-FFT tile that the operator uses for L columns."""
+        lb = self.lib.bdda_plan_column_tile(self.h, int(L))
+        if lb < 1:
+            raise ValueError("column_tile: bad L")
+        return lb
+
     def green_tensor(self, lf, k):
         out = np.empty(self.m + (3, 3), dtype=np.complex128)
         check(self.lib.bdda_green_tensor(self.h, float(lf), float(k), ptr(out)), "green_tensor")

@@ -2,9 +2,9 @@
 
 Usage:  python block-dda_py_amd/build_lib.py [--jobs N]
 
-Each csrc/*.hip file is compiled with hipcc for gfx950 in parallel, then the
-objects are linked into one shared library exporting the C ABI declared in
-include/bdda.h.
+Each csrc/*.hip file is compiled with hipcc for gfx950 and each host-only
+csrc/*.cpp file with g++, in parallel; the objects are linked into one shared
+library exporting the C ABI declared in include/bdda.h.
 """
 import argparse
 import concurrent.futures as cf
@@ -27,21 +27,27 @@ FLAGS = [
     "-I", os.path.join(ROOT, "include"),
 ]
 SOURCES = ["kernels_fft.hip", "kernels_block.hip", "kernels_fno.hip", "plan.hip"]
+# host-only C++ (built with the host compiler)
+CXX_SOURCES = ["host_la.cpp"]
+CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC"]
 
 
 def _compile(src):
     obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".o")
-    cmd = ["hipcc", *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    if src.endswith(".cpp"):
+        cmd = ["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    else:
+        cmd = ["hipcc", *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")
+        raise RuntimeError(f"{cmd[0]} failed for {src}:\n{r.stderr}")
     return obj
 
 
 def build(jobs=4, verbose=True):
     os.makedirs(OBJ_DIR, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        objs = list(ex.map(_compile, SOURCES + CXX_SOURCES))
     tmp = LIB + ".tmp"
     cmd = ["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)

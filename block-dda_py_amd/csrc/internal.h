@@ -86,8 +86,9 @@ struct LinePass {
 
 template <typename V>
 void launch_line_pass(const LinePass& lp, hipStream_t s);
-// Route every length through the runtime-radix kernel (tests compare both).
-void set_force_generic_fft(bool on);
+// FFT kernel selection (tests compare the paths): 0 register kernels where
+// a length has one, 1 the runtime-radix kernel only.
+void set_fft_path(int path);
 template <typename V>
 void launch_green_build(V* out, const AxisPlan* ax, double lf, double k, hipStream_t s);
 template <typename V>

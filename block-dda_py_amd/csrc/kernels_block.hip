@@ -90,18 +90,27 @@ __global__ void __launch_bounds__(256) k_gram(const V* __restrict__ A, const V* 
     for (int j = 0; j < 4; ++j) out[(ty * 4 + i) * 64 + tx * 4 + j] = acc[i][j];
 }
 
-__global__ void k_gram_reduce(const double2* __restrict__ partial, int nparts, int ntile2, int L,
-                              double2* __restrict__ out) {
+// out[j][l] = sum of the parts' 64 x 64 tiles.  A block takes 16 consecutive
+// tile entries and 16 part groups: group g sums parts g, g + 16, ..., and the
+// 16 group sums are added in group order (a fixed order: deterministic).
+// Entries outside the L x L result are skipped.
+constexpr int kRedEntries = 16, kRedGroups = 16;
+__global__ void __launch_bounds__(256) k_gram_reduce(const double2* __restrict__ partial, int nparts, int ntile2,
+                                                     int L, double2* __restrict__ out) {
+  __shared__ double2 sh[kRedGroups][kRedEntries];
+  const int e = threadIdx.x % kRedEntries, g = threadIdx.x / kRedEntries;
+  const long long id = (long long)blockIdx.x * kRedEntries + e;
   const int ntile = (L + 63) / 64;
-  const long long total = (long long)ntile2 * 4096;
-  for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
-       id += (long long)gridDim.x * blockDim.x) {
-    const int tile = id / 4096, w = id % 4096;
-    const int jt = tile / ntile, lt = tile % ntile;
-    const int j = jt * 64 + w / 64, l = lt * 64 + w % 64;
-    if (j >= L || l >= L) continue;
-    double2 s = make_double2(0.0, 0.0);
-    for (int p = 0; p < nparts; ++p) s = zadd(s, partial[((long long)p * ntile2 + tile) * 4096 + w]);
+  const int tile = (int)(id / 4096), w = (int)(id % 4096);
+  const int j = (tile / ntile) * 64 + w / 64, l = (tile % ntile) * 64 + w % 64;
+  const bool live = id < (long long)ntile2 * 4096 && j < L && l < L;
+  double2 s = make_double2(0.0, 0.0);
+  if (live)
+    for (int p = g; p < nparts; p += kRedGroups) s = zadd(s, partial[((long long)p * ntile2 + tile) * 4096 + w]);
+  sh[g][e] = s;
+  __syncthreads();
+  if (g == 0 && live) {
+    for (int q = 1; q < kRedGroups; ++q) s = zadd(s, sh[q][e]);
     out[j * L + l] = s;
   }
 }
@@ -115,7 +124,9 @@ void launch_gram(const V* A, const V* B, long long rows, int L, bool conjA, doub
     hipLaunchKernelGGL((k_gram<V, true>), grid, dim3(256), 0, s, A, B, rows, L, partial);
   else
     hipLaunchKernelGGL((k_gram<V, false>), grid, dim3(256), 0, s, A, B, rows, L, partial);
-  hipLaunchKernelGGL(k_gram_reduce, dim3(64), dim3(256), 0, s, partial, nparts, ntile * ntile, L, out);
+  const long long entries = (long long)ntile * ntile * 4096;
+  hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)((entries + kRedEntries - 1) / kRedEntries)),
+                     dim3(kRedEntries * kRedGroups), 0, s, partial, nparts, ntile * ntile, L, out);
 }
 template void launch_gram<double2>(const double2*, const double2*, long long, int, bool, double2*, int, double2*,
                                    hipStream_t);

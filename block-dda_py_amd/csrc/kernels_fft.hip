@@ -833,7 +833,7 @@ void launch_line_pass(const LinePass& lp0, hipStream_t s) {
   }
 }
 
-void set_force_generic_fft(bool on) { g_force_generic = on; }
+void set_fft_path(int path) { g_force_generic = path == 1; }
 
 template void launch_line_pass<double2>(const LinePass&, hipStream_t);
 template void launch_line_pass<float2>(const LinePass&, hipStream_t);

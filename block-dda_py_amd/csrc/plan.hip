@@ -143,8 +143,9 @@ template <> struct PrecTraits<float2> {
 };
 
 // Column tile Lb: every line-FFT tile (m x 3 Lb complex) must fit 4096
-// values; among the admissible tiles pick the one that pads L least, then the
-// widest (longer contiguous runs per grid point).
+// values; among the admissible tiles pick the one that pads L least, then
+// the 8-column tile (the register kernels have a compile-time version of it),
+// then the widest (longer contiguous runs per grid point).
 static int choose_lb(const bdda_plan* p, int L) {
   const int mmax = std::max(p->ax[0].m, std::max(p->ax[1].m, p->ax[2].m));
   int cap = 4096 / (3 * mmax);
@@ -153,7 +154,7 @@ static int choose_lb(const bdda_plan* p, int L) {
   long long best_pad = (long long)L;
   for (int lb = cap; lb >= 1; --lb) {
     const long long pad = (long long)((L + lb - 1) / lb) * lb;
-    if (pad < best_pad || (pad == best_pad && lb > best)) {
+    if (pad < best_pad || (pad == best_pad && best != 8 && (lb == 8 || lb > best))) {
       best_pad = pad;
       best = lb;
     }
@@ -383,26 +384,48 @@ static bool stalled(const Ctx& c) {
   return recent > 0.9 * before;
 }
 
-static int gram_to_host(Ctx& c, const void* A, const void* B, bool conj, bool f32, std::vector<zc>& out) {
+// Gram matrices op(A)^H B (conj) or A^T B into device / pinned host slot
+// `slot` (0..2) without waiting: several Grams (and a norm) then share one
+// stream synchronisation, gram_unpack reads a slot after host_wait.
+constexpr int kGramSlots = 3;
+static int gram_launch(Ctx& c, const void* A, const void* B, bool conj, bool f32, int slot) {
   bdda_plan* p = c.p;
   const int L = c.L;
   const int ntile = (L + 63) / 64;
   const int np = gram_parts();
+  const size_t one = (size_t)L * L * sizeof(double2);
   int rc = p->gram_part.ensure((size_t)np * ntile * ntile * 4096 * sizeof(double2));
-  if (!rc) rc = p->gram_out.ensure((size_t)L * L * sizeof(double2));
-  if (!rc) rc = p->h_gram.ensure((size_t)L * L * sizeof(double2));
+  if (!rc) rc = p->gram_out.ensure(kGramSlots * one);
+  if (!rc) rc = p->h_gram.ensure(kGramSlots * one);
   if (rc) return rc;
+  double2* dout = p->gram_out.as<double2>() + (size_t)slot * L * L;
   if (f32)
-    launch_gram<float2>((const float2*)A, (const float2*)B, c.rows, L, conj, p->gram_part.as<double2>(), np,
-                        p->gram_out.as<double2>(), c.s);
+    launch_gram<float2>((const float2*)A, (const float2*)B, c.rows, L, conj, p->gram_part.as<double2>(), np, dout,
+                        c.s);
   else
     launch_gram<double2>((const double2*)A, (const double2*)B, c.rows, L, conj, p->gram_part.as<double2>(), np,
-                         p->gram_out.as<double2>(), c.s);
+                         dout, c.s);
   BDDA_HIP(hipGetLastError());
-  BDDA_HIP(hipMemcpyAsync(p->h_gram.p, p->gram_out.p, (size_t)L * L * sizeof(double2), hipMemcpyDeviceToHost, c.s));
+  BDDA_HIP(hipMemcpyAsync(p->h_gram.as<char>() + slot * one, dout, one, hipMemcpyDeviceToHost, c.s));
+  return 0;
+}
+
+static int host_wait(Ctx& c) {
   BDDA_HIP(hipStreamSynchronize(c.s));
-  out.resize((size_t)L * L);
-  std::memcpy(out.data(), p->h_gram.p, (size_t)L * L * sizeof(double2));
+  return 0;
+}
+
+static void gram_unpack(Ctx& c, int slot, std::vector<zc>& out) {
+  const size_t one = (size_t)c.L * c.L * sizeof(double2);
+  out.resize((size_t)c.L * c.L);
+  std::memcpy(out.data(), c.p->h_gram.as<char>() + slot * one, one);
+}
+
+static int gram_to_host(Ctx& c, const void* A, const void* B, bool conj, bool f32, std::vector<zc>& out) {
+  int rc = gram_launch(c, A, B, conj, f32, 0);
+  if (!rc) rc = host_wait(c);
+  if (rc) return rc;
+  gram_unpack(c, 0, out);
   return 0;
 }
 
@@ -452,8 +475,10 @@ static int dots_to_host(Ctx& c, const V* A, const V* B, zc* dot, double* nrm) {
   return 0;
 }
 
+// Y = X + a Z with |Y|_F^2 partial sums copied to the host (no wait);
+// axpy_norm_sum adds them up after host_wait.
 template <typename V>
-static int axpy_norm_host(Ctx& c, V* Y, const V* X, zc a, const V* Z, double* nrm2) {
+static int axpy_norm_launch(Ctx& c, V* Y, const V* X, zc a, const V* Z) {
   bdda_plan* p = c.p;
   const int np = dot_parts();
   int rc = p->dots_n.ensure(np * sizeof(double));
@@ -461,12 +486,23 @@ static int axpy_norm_host(Ctx& c, V* Y, const V* X, zc a, const V* Z, double* nr
   if (rc) return rc;
   launch_axpy_norm<V>(Y, X, make_double2(a.real(), a.imag()), Z, c.rows * c.L, p->dots_n.as<double>(), np, c.s);
   BDDA_HIP(hipGetLastError());
-  double* hn = p->h_dots.as<double>();
-  BDDA_HIP(hipMemcpyAsync(hn, p->dots_n.p, np * sizeof(double), hipMemcpyDeviceToHost, c.s));
-  BDDA_HIP(hipStreamSynchronize(c.s));
+  BDDA_HIP(hipMemcpyAsync(p->h_dots.p, p->dots_n.p, np * sizeof(double), hipMemcpyDeviceToHost, c.s));
+  return 0;
+}
+
+static double axpy_norm_sum(Ctx& c) {
+  const double* hn = c.p->h_dots.as<double>();
   double s = 0.0;
-  for (int i = 0; i < np; ++i) s += hn[i];
-  *nrm2 = s;
+  for (int i = 0; i < dot_parts(); ++i) s += hn[i];
+  return s;
+}
+
+template <typename V>
+static int axpy_norm_host(Ctx& c, V* Y, const V* X, zc a, const V* Z, double* nrm2) {
+  int rc = axpy_norm_launch<V>(c, Y, X, a, Z);
+  if (!rc) rc = host_wait(c);
+  if (rc) return rc;
+  *nrm2 = axpy_norm_sum(c);
   return 0;
 }
 
@@ -524,8 +560,11 @@ static int solve_bicgstab(Ctx& c, const V* Bj, V* X, bool x_zero, double bnorm, 
   double err = INFINITY;
   for (int k = 0; k < itermax; ++k) {
     if ((rc = op<V>(c, P, Vb))) return rc;
-    if ((rc = gram_to_host(c, R0, Vb, true, is_f32<V>(), RV))) return rc;
-    if ((rc = gram_to_host(c, R0, R, true, is_f32<V>(), RR))) return rc;
+    if ((rc = gram_launch(c, R0, Vb, true, is_f32<V>(), 0)) || (rc = gram_launch(c, R0, R, true, is_f32<V>(), 1)) ||
+        (rc = host_wait(c)))
+      return rc;
+    gram_unpack(c, 0, RV);
+    gram_unpack(c, 1, RR);
     std::vector<zc> LU = RV;
     if (!la::lu_factor(LU, piv, L)) return singular("block-BiCGSTAB (R0^H V)");
     alpha = RR;
@@ -535,21 +574,23 @@ static int solve_bicgstab(Ctx& c, const V* Bj, V* X, bool x_zero, double bnorm, 
     // T = R - V alpha
     if ((rc = update<V>(c, T, R, 1.0, Vb, nullptr, 0.0, negA, 0, nullptr, 0.0))) return rc;
     if ((rc = op<V>(c, T, Z))) return rc;
-    zc zt, zz;
+    // xi = tr(Z^H T) / tr(Z^H Z) from one pass: sum conj(T) Z and |Z|^2
+    zc tz;
     double zn2;
-    if ((rc = dots_to_host<V>(c, Z, T, &zt, nullptr))) return rc;
-    if ((rc = dots_to_host<V>(c, Z, Z, &zz, &zn2))) return rc;
-    const zc xi = zt / zz;
+    if ((rc = dots_to_host<V>(c, T, Z, &tz, &zn2))) return rc;
+    const zc xi = std::conj(tz) / zn2;
     // X = X + P alpha + xi T
     if ((rc = update<V>(c, X, X, 1.0, P, nullptr, 0.0, alpha, 1, T, xi))) return rc;
-    // R = T - xi Z
-    double rn2;
-    if ((rc = axpy_norm_host<V>(c, R, T, -xi, Z, &rn2))) return rc;
+    // R = T - xi Z with its norm, and R0^H Z for the next beta, one wait
+    if ((rc = axpy_norm_launch<V>(c, R, T, -xi, Z)) || (rc = gram_launch(c, R0, Z, true, is_f32<V>(), 2)) ||
+        (rc = host_wait(c)))
+      return rc;
+    const double rn2 = axpy_norm_sum(c);
     err = std::sqrt(rn2) / bnorm;
     if (c.hist) c.hist->push_back(err);
     it = k;
     if (err < tol || stalled(c)) break;
-    if ((rc = gram_to_host(c, R0, Z, true, is_f32<V>(), RZ))) return rc;
+    gram_unpack(c, 2, RZ);
     beta = RZ;
     for (auto& v : beta) v = -v;
     la::lu_solve(LU, piv, L, beta, L);
@@ -697,7 +738,7 @@ int bdda_set_fft_path(int path) {
     set_error("bdda_set_fft_path: path must be 0 (auto) or 1 (runtime-radix only)");
     return BDDA_E_ARG;
   }
-  set_force_generic_fft(path == 1);
+  set_fft_path(path);
   return 0;
 }
 
@@ -755,6 +796,11 @@ int bdda_plan_fft_dims(const bdda_plan* plan, int* m_out) {
   if (int rc = check_plan(plan)) return rc;
   for (int a = 0; a < 3; ++a) m_out[a] = plan->ax[a].m;
   return 0;
+}
+
+int bdda_plan_column_tile(const bdda_plan* p, int L) {
+  if (check_plan(p) || L < 1) return -BDDA_E_ARG;
+  return choose_lb(p, L);
 }
 
 int bdda_green_init(bdda_plan* p, double lf, double k) {

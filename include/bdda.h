@@ -57,6 +57,11 @@ int bdda_plan_destroy(bdda_plan* plan);
 /* FFT lengths per axis (3 ints). */
 int bdda_plan_fft_dims(const bdda_plan* plan, int* m_out);
 
+/* Column tile Lb (orientations per line-FFT tile) the operator uses for L
+ * columns: the FFT passes move ceil(L / Lb) * Lb columns.  Returns Lb, or
+ * -BDDA_E_ARG.  For measurement (byte counts); no reference counterpart. */
+int bdda_plan_column_tile(const bdda_plan* plan, int L);
+
 /* Green tensor A'(r) on the doubled grid and its 3-D FFT, kept on device in
  * complex128 and complex64 (6 unique components, half spectrum).
  * Replaces build_interaction_tensor + fft_init (mvp_fft/mvp_fft.py:13-119). */

@@ -78,6 +78,16 @@ def test_register_and_stockham_kernels_agree(n):
         assert rel(out[path][1], ref) < 2e-6
 
 
+def test_column_tile_prefers_the_compiled_8_column_tile():
+    """Among equally padded tiles the operator takes 8 columns (the register
+    kernels' compile-time tile); a tile that pads L less wins otherwise."""
+    from bdda_amd.plan import Plan
+    assert Plan((32, 9, 4)).column_tile(16) == 8   # 16 columns would also fit
+    assert Plan((64, 6, 5)).column_tile(64) == 8
+    assert Plan((64, 6, 5)).column_tile(70) == 10  # no padding
+    assert Plan((64, 6, 5)).column_tile(5) == 5
+
+
 def test_wide_column_tile_on_register_lengths():
     """L = 64 on a small grid selects 16-column tiles (Q = 48); lengths 48
     (16 x 3) and 80 (16 x 5) would need 768 threads in the register kernel
