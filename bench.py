@@ -28,6 +28,33 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def copy_calibration():
+    """A plain device copy of known size (2 GiB read, 2 GiB written, three
+    times) for scaling the FETCH_SIZE / WRITE_SIZE counters of rocprofv3."""
+    import torch
+    n = 1 << 28
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_each": n * 8, "copies": 3}), flush=True)
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "x_conv_f64_gre64_L64_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the dominant kernel as measured with rocprofv3
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the copy calibration), read from
+    its committed summary; None when there is none."""
+    import re
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    m = re.search(r"traffic per launch:\s*([0-9]+)\s*bytes", open(TRAFFIC_FILE).read())
+    return int(m.group(1)) if m else None
+
+
 def pass_bytes(dims, m, N, L, Lb, s):
     """Algorithmic HBM bytes of each FFT pass for one product (each pass reads
     and writes its pruned array once; the x pass also reads the 6-component
@@ -159,7 +186,11 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true", help="only run the copy used to scale HBM counters")
     a = ap.parse_args()
+    if a.copy_calibration:
+        copy_calibration()
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU layout is a design note, "
               "not a measured path", file=sys.stderr)
@@ -199,9 +230,14 @@ def main():
     M = int(np.prod(m))
     mvp_alg = 16 * (26 * n_cells * 3 * L + 6 * M)
     mvp_ms = sum(p["avg_ms"] for p in passes.values())
-    roofline = {"bound": "hbm", "kernel": f"k_line_pass<double2, {dom}>", "achieved": round(d["GBs"], 1),
+    kernel = {"x_conv": "x pass (forward x FFT, 3x3 Green multiply, inverse x FFT)", "y_fwd": "y forward pass",
+              "y_inv": "y inverse pass", "z_fwd": "z forward pass (scatter)",
+              "z_inv": "z inverse pass (gather)"}[dom]
+    roofline = {"bound": "hbm", "kernel": f"{kernel}, m = {m[0]}, complex128, {Lb}-column tile",
+                "achieved": round(d["GBs"], 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(d["GBs"] / HBM_PEAK_GBS, 4),
-                "traffic": None, "bytes_per_launch": d["bytes"], "avg_launch_ms": round(d["avg_ms"], 4)}
+                "traffic": measured_traffic(), "bytes_per_launch": d["bytes"],
+                "avg_launch_ms": round(d["avg_ms"], 4)}
 
     result = {
         "metric": METRIC, "value": round(L * a.steps / dt, 3), "unit": "orientations/s", "n_gpus": 1,

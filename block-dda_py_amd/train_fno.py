@@ -73,6 +73,51 @@ def gain(model, n_eval=4, L=64, G=32, seed=12345):
     return rows
 
 
+def guess_study(n_inst=2, L=64, G=32, seed=12345, levels=(0.3, 0.1, 0.03, 0.01, 1e-3)):
+    """Iterations to 1e-5 from perturbed solutions: how much an initial guess
+    of a given relative error can save (block-COCG-rq, float64 and mixed, on
+    held-out isotropic particles).  The perturbation is random complex noise
+    or a smooth per-orientation scale and phase error of the stated norm."""
+    import numpy as np
+    import torch
+    from bdda_amd.fno_train import ShapeBank, random_dipoles
+    from bl_dda.scatterer import DiscreteDipoles, IncidentField, Target
+    rng = np.random.default_rng(seed)
+    bank = ShapeBank(seed + 1)
+    rows = []
+    for inst in range(n_inst):
+        d0 = random_dipoles(rng, bank, G, L)
+        tg = Target("eval", d0.lattice_n, d0.lattice_lf, d0.lattice_grid_points,
+                    d0.lattice_grid_points_is_in_target, np.array([d0.m_p_xyz[0]] * 3))
+        dd = DiscreteDipoles(tg, IncidentField(d0.wl_0, d0.m_m, d0.euler_angles))
+        dd.verbose = False
+        dd.set_interaction_matrix()
+        dd.solver, dd.precision, dd.tol, dd.itermax = "cocg_rq", "fp64", 1e-10, 300
+        dd.solve_matrix_equation()
+        Xs = dd.X_device.clone()
+        dd.tol = 1e-5
+        for prec in ("fp64", "mixed"):
+            dd.precision, dd.X0 = prec, None
+            dd.solve_matrix_equation()
+            row = dict(instance=inst, precision=prec, zero_start=dd.iter_fin + 1)
+            g = torch.Generator(device="cuda")
+            g.manual_seed(0)
+            for kind in ("random", "smooth"):
+                for d in levels:
+                    if kind == "random":
+                        E = torch.randn(Xs.shape, dtype=torch.complex128, device="cuda", generator=g)
+                    else:
+                        c = 1 + d * torch.randn((1, Xs.shape[1]), dtype=torch.complex128, device="cuda", generator=g)
+                        E = Xs * (c - 1)
+                    E = E / torch.linalg.norm(E) * torch.linalg.norm(Xs) * d
+                    dd.X0 = Xs + E
+                    dd.solve_matrix_equation()
+                    row[f"{kind}_{d:g}"] = dd.iter_fin + 1
+            log(f"guess study: {row}")
+            rows.append(row)
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--instances", type=int, default=2000)
@@ -84,11 +129,15 @@ def main():
     ap.add_argument("--eval", type=int, default=4)
     ap.add_argument("--out", default=os.path.join(HERE, "bdda_amd", "data", "fno_gre32.npz"))
     ap.add_argument("--eval-only", action="store_true", help="only measure the gain of the weights at --out")
+    ap.add_argument("--guess-study", action="store_true", help="iterations saved by guesses of known error")
     a = ap.parse_args()
     from bdda_amd.fno import FNO
     from bdda_amd.fno_train import generate, train
     from bl_dda.scatterer import DiscreteDipoles
     DiscreteDipoles.verbose_default = False
+    if a.guess_study:
+        print(json.dumps(dict(guess_study=guess_study())), flush=True)
+        return
     if a.eval_only:
         model = FNO.load(a.out)
         print(json.dumps(dict(weights=a.out, gain=gain(model, n_eval=a.eval, G=a.grid))), flush=True)
