@@ -7,7 +7,8 @@ beta = 0.2 (seed 0) on a 51 x 48 x 63 lattice padded to a 64^3 cuboid
 m_m = 1.33, L = 64 orientations, complex128 block-BiCGSTAB to relative
 residual 1e-5 with B resident on the device.  Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-mixed] [--no-batch] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-mixed] [--no-large] [--no-batch]
+                    [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -127,6 +128,29 @@ def cpu_baseline(geometry, wl, gpu_iters, sample_L=16):
                       f"{wl['L'] // sample_L} in L and x{gpu_iters} iterations (the GPU float64 count)"}
 
 
+def large_leg(torch, steps=2):
+    """The 96^3 configuration (config-1 physics at four times the size,
+    L = 32, block-COCG-rq to 1e-5): float64 against mixed precision, with
+    the S errors of the mixed solve relative to the float64 one."""
+    import numpy as np
+    from bdda_amd import workloads
+    wl = workloads.LARGE_96
+    dd = workloads.make_dipoles(wl, precision="fp64")
+    out = {"workload": wl["name"], "lattice": [int(v) for v in dd.lattice_n], "cuboid": list(dd._plan.n),
+           "fft_grid": list(dd._plan.m), "dipoles": int(dd.num_element_occupy), "L": wl["L"],
+           "solver": wl["solver"], "tol": wl["tol"], "unit": "orientations/s"}
+    S = {}
+    for prec in ("fp64", "mixed"):
+        dt, _ = timed_solves(dd, prec, steps, 1, torch, profile_steps=0)
+        S[prec] = (np.stack(dd.compute_PCAS_observable_S_fw()), dd.compute_OCBS_observable_S_bk())
+        out[prec] = {"value": round(wl["L"] * steps / dt, 2), "ms_per_solve": round(dt / steps * 1e3, 2),
+                     "iterations": dd.iter_fin + 1, "true_residual": dd.true_residual}
+    out["mixed"]["speedup_vs_fp64"] = round(out["mixed"]["value"] / out["fp64"]["value"], 3)
+    out["mixed"]["S_fw_max_rel_err_vs_fp64"] = float(np.max(np.abs(S["mixed"][0] - S["fp64"][0]) / np.abs(S["fp64"][0])))
+    out["mixed"]["S_bk_max_rel_err_vs_fp64"] = float(np.max(np.abs(S["mixed"][1] - S["fp64"][1]) / np.abs(S["fp64"][1])))
+    return out
+
+
 def batch_leg(torch, nstreams=8):
     """Batched uneven instances (32 GRE particles, lattices 20..48 cells,
     L = 16, block-COCG-rq to 1e-5): per precision, one-at-a-time solves
@@ -186,6 +210,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true", help="only run the copy used to scale HBM counters")
     a = ap.parse_args()
     if a.copy_calibration:
@@ -275,6 +300,12 @@ def main():
             "passes_c64": mp,
         }
         log(f"mixed: {dtm / a.steps * 1e3:.1f} ms/solve")
+
+    if not a.no_large:
+        log("96^3 configuration")
+        result["large"] = large_leg(torch)
+        g = result["large"]
+        log(f"96^3: fp64 {g['fp64']['ms_per_solve']} ms, mixed {g['mixed']['ms_per_solve']} ms")
 
     if not a.no_batch:
         log("batched uneven instances")

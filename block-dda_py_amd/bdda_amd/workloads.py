@@ -27,6 +27,18 @@ SMALL_24 = dict(
 )
 
 
+# Config-1 physics (bc = ab = 1, beta = 0.1, seed 0, m_p = 1.5 + 0.01i, m_m = 1)
+# scaled to a 96^3 cuboid at the same resolution: r_v = 6.2 um gives a
+# ~91 x 92 x 87 lattice (lf = 0.148 um) padded to 96^3 (192^3 FFT grid),
+# size parameter 12 (about 16 dipoles per wavelength, as config 1's 14);
+# L = 32; mixed-precision block-COCG-rq to 1e-5.
+LARGE_96 = dict(
+    name="gre_bc1_ab1_beta0.1_seed0_96cube_L32",
+    r_v_base=6.2, bc_ratio=1.0, ab_ratio=1.0, beta=0.1, shape_seed=0,
+    m_p_xyz=(1.5 + 0.01j,) * 3, m_m=1.0, size_parameter=12.0,
+    L=32, orient_seed=1, pad_to=(96, 96, 96), solver="cocg_rq", tol=1e-5, itermax=500,
+)
+
 # Batched uneven instances: 32 GRE particles (shape seeds 0-31) whose longest
 # lattice axis is drawn uniformly from 20..48 cells, bc and ab uniform in
 # [0.5, 1], beta uniform in [0, 0.3], isotropic m_p with real part uniform in

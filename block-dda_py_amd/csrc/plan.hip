@@ -362,27 +362,15 @@ struct Ctx {
   long long rows;
   hipStream_t s;
   std::vector<double>* hist;
-  // inner solves of the mixed mode end early once they stall, or when their
-  // block QR breaks down, so that the float64 refinement restarts them from
-  // the true residual
-  bool stop_on_stall = false;
+  // mixed mode: every check_every iterations the true residual is computed
+  // with the complex128 operator; once it exceeds twice the recursive one
+  // (or is not finite) the solve stops with conv32 cleared, and the caller
+  // restarts from the true residual with the complex128 convolution
+  int check_every = 0;
   // mixed mode: the operator's convolution runs in complex64 (complex128
   // blocks at its scatter / gather boundaries)
   bool conv32 = false;
 };
-
-// No real progress over the last kStallWindow iterations: their best
-// residual is not 10% below the best one before them (block COCG residuals
-// rise and fall, so a shorter window would cut healthy iterations short).
-constexpr int kStallWindow = 6;
-static bool stalled(const Ctx& c) {
-  if (!c.stop_on_stall || !c.hist || (int)c.hist->size() <= kStallWindow + 1) return false;
-  const std::vector<double>& h = *c.hist;
-  const size_t split = h.size() - kStallWindow;
-  const double before = *std::min_element(h.begin(), h.begin() + split);
-  const double recent = *std::min_element(h.begin() + split, h.end());
-  return recent > 0.9 * before;
-}
 
 // Gram matrices op(A)^H B (conj) or A^T B into device / pinned host slot
 // `slot` (0..2) without waiting: several Grams (and a norm) then share one
@@ -520,6 +508,15 @@ static int update(Ctx& c, V* Y, const V* X, zc a, const V* W1, const V* W2, zc w
 
 template <typename V> static constexpr bool is_f32() { return sizeof(V) == sizeof(float2); }
 
+// Mixed mode, at iteration k of a complex64-convolution cycle: keep going
+// only while the recursive residual err is on a log-linear course to tol by
+// the next check (err <= tol^(k / (k + every))); a solve that is not is
+// hard, and complex64 rounding in its products delays convergence for good
+// (lost conjugacy), so the caller restarts it in complex128.
+static bool conv32_on_track(double err, double tol, int k, int every) {
+  return err <= std::pow(tol, (double)k / (double)(k + every));
+}
+
 template <typename V>
 static int op(Ctx& c, const V* S, V* AS) {
   if constexpr (std::is_same<V, double2>::value) {
@@ -559,6 +556,24 @@ static int solve_bicgstab(Ctx& c, const V* Bj, V* X, bool x_zero, double bnorm, 
   int it = 0;
   double err = INFINITY;
   for (int k = 0; k < itermax; ++k) {
+    if constexpr (std::is_same<V, double2>::value) {
+      if (c.conv32 && c.check_every > 0 && k > 0 && k % c.check_every == 0) {
+        // stop (the caller restarts in complex128) when the solve is off
+        // course, or when the true residual, from the complex128 operator,
+        // has drifted from the recursive one
+        bool stop = !conv32_on_track(err, tol, k, c.check_every);
+        if (!stop) {
+          if ((rc = apply_operator<double2>(c.p, X, T, L, true, c.s))) return rc;
+          double rt2;
+          if ((rc = axpy_norm_host<V>(c, Z, Bj, zc(-1.0), T, &rt2))) return rc;
+          stop = !(std::sqrt(rt2) / bnorm <= 2.0 * err);
+        }
+        if (stop) {
+          c.conv32 = false;
+          break;
+        }
+      }
+    }
     if ((rc = op<V>(c, P, Vb))) return rc;
     if ((rc = gram_launch(c, R0, Vb, true, is_f32<V>(), 0)) || (rc = gram_launch(c, R0, R, true, is_f32<V>(), 1)) ||
         (rc = host_wait(c)))
@@ -589,7 +604,7 @@ static int solve_bicgstab(Ctx& c, const V* Bj, V* X, bool x_zero, double bnorm, 
     err = std::sqrt(rn2) / bnorm;
     if (c.hist) c.hist->push_back(err);
     it = k;
-    if (err < tol || stalled(c)) break;
+    if (err < tol) break;
     gram_unpack(c, 2, RZ);
     beta = RZ;
     for (auto& v : beta) v = -v;
@@ -674,6 +689,24 @@ static int solve_cocg(Ctx& c, const V* Bj, V* X, bool x_zero, double bnorm, doub
   int it = 0;
   double err = INFINITY;
   for (int k = 0; k < itermax; ++k) {
+    if constexpr (std::is_same<V, double2>::value) {
+      if (c.conv32 && c.check_every > 0 && k > 0 && k % c.check_every == 0) {
+        // stop (the caller restarts in complex128) when the solve is off
+        // course, or when the true residual, from the complex128 operator,
+        // has drifted from the recursive one
+        bool stop = !conv32_on_track(err, tol, k, c.check_every);
+        if (!stop) {
+          if ((rc = apply_operator<double2>(c.p, X, AS, L, true, c.s))) return rc;
+          double rt2;
+          if ((rc = axpy_norm_host<V>(c, W, Bj, zc(-1.0), AS, &rt2))) return rc;
+          stop = !(std::sqrt(rt2) / bnorm <= 2.0 * err);
+        }
+        if (stop) {
+          c.conv32 = false;
+          break;
+        }
+      }
+    }
     if ((rc = op<V>(c, S, AS))) return rc;
     if ((rc = gram_to_host(c, S, AS, false, is_f32<V>(), StAS))) return rc;
     std::vector<zc> LU = StAS;
@@ -696,7 +729,7 @@ static int solve_cocg(Ctx& c, const V* Bj, V* X, bool x_zero, double bnorm, doub
     err = la::frob(xi) / bnorm;
     if (c.hist) c.hist->push_back(err);
     it = k;
-    if (err < tol || stalled(c)) break;
+    if (err < tol) break;
     if ((rc = gram_to_host(c, Qn, Qn, false, is_f32<V>(), QntQn))) return rc;
     la::matmul(tau, true, QntQn, rhs, L);  // tau^T (Qn^T Qn)
     LU = QtQ;
@@ -1052,13 +1085,14 @@ int bdda_solve(bdda_plan* p, int solver, int precision, const void* B_dev, const
     // mixed precision: the Goodman convolution (nearly all of the operator's
     // bytes) runs in complex64 between complex128 scatter / gather
     // boundaries; the Krylov blocks, their updates, Gram matrices and L x L
-    // solves stay float64.  Cycles of at most refine_every iterations solve
-    // for the correction D on the float64 true residual R = Bj - A X, then
-    // X += D and R is recomputed with the float64 operator.
-    // A cycle that does not halve the true residual (the Krylov recurrence
-    // amplifying the complex64 rounding on a hard problem) hands the
-    // remaining cycles to the complex128 convolution; one that makes the
-    // residual worse is undone first.
+    // solves stay float64.  A cycle solves for the correction D on the
+    // float64 true residual R = Bj - A X with the complex64 convolution;
+    // every refine_every iterations it computes its true residual with the
+    // complex128 operator, and once that exceeds twice the recursive one
+    // (complex64 rounding amplified by the recurrence) it stops.  After a
+    // cycle X += D and R is recomputed; a cycle that made R worse is undone.
+    // Any further cycle restarts from the true residual with the complex128
+    // convolution and the remaining iteration budget.
     if ((rc = p->blk[9].ensure(b64)) || (rc = p->blk[10].ensure(b64))) return rc;
     double2* ws[7];
     for (int i = 0; i < 7; ++i) ws[i] = p->blk[i].as<double2>();
@@ -1077,24 +1111,24 @@ int bdda_solve(bdda_plan* p, int solver, int precision, const void* B_dev, const
     double r2;
     if ((rc = frob2(c, R64, &r2))) return rc;
     double res = std::sqrt(r2) / bnorm;
-    const int cycle = refine_every > 0 ? refine_every : itermax;
     int total = 0;
     err = res;
     while (res >= tol && total < itermax) {
       const double rs = std::sqrt(r2);
-      // the complex64 convolution bounds what a cycle can reach
+      // the complex64 convolution's 1e-7 rounding bounds what a cycle reaches
       double inner_tol = 0.5 * tol * bnorm / rs;
       if (inner_tol < 2e-6) inner_tol = 2e-6;
       std::vector<double> inner;
       c.hist = &inner;
-      c.stop_on_stall = true;
       c.conv32 = conv32;
+      c.check_every = refine_every > 0 ? refine_every : 0;
+      const bool was32 = conv32;
       int iit = 0;
       double ierr = 0.0;
-      const int budget = std::min(cycle, itermax - total);
-      rc = run_solver<double2>(c, solver, R64, D, true, rs, inner_tol, budget, ws, &iit, &ierr);
+      rc = run_solver<double2>(c, solver, R64, D, true, rs, inner_tol, itermax - total, ws, &iit, &ierr);
+      conv32 = c.conv32;
       c.conv32 = false;
-      c.stop_on_stall = false;
+      c.check_every = 0;
       c.hist = &hist;
       // a singular block QR / L x L system ends the cycle; the solvers update
       // D in place, so the correction reached so far stands
@@ -1110,21 +1144,24 @@ int bdda_solve(bdda_plan* p, int solver, int precision, const void* B_dev, const
       if (rc) return rc;
       for (double e : inner) hist.push_back(e * rs / bnorm);
       total += (int)inner.size();
-      if (conv32) BDDA_HIP(hipMemcpyAsync(Rsave, R64, b64, hipMemcpyDeviceToDevice, s));
+      // the Krylov blocks are free between cycles: ws[0] keeps X for an undo
+      // (restored by copy, so a correction that is not finite is undone too)
+      if (was32) {
+        BDDA_HIP(hipMemcpyAsync(Rsave, R64, b64, hipMemcpyDeviceToDevice, s));
+        BDDA_HIP(hipMemcpyAsync(ws[0], X, b64, hipMemcpyDeviceToDevice, s));
+      }
       launch_axpy_norm<double2>(X, X, make_double2(1.0, 0.0), D, n, nullptr, dot_parts(), s);
       BDDA_HIP(hipGetLastError());
       if ((rc = apply_operator<double2>(p, X, tmp, L, true, s))) return rc;
       double r2new;
       if ((rc = axpy_norm_host<double2>(c, R64, Bj, zc(-1.0), tmp, &r2new))) return rc;
-      if (conv32 && r2new > 0.25 * r2) {
-        conv32 = false;
-        if (r2new >= r2) {  // undo the cycle
-          launch_axpy_norm<double2>(X, X, make_double2(-1.0, 0.0), D, n, nullptr, dot_parts(), s);
-          BDDA_HIP(hipGetLastError());
-          BDDA_HIP(hipMemcpyAsync(R64, Rsave, b64, hipMemcpyDeviceToDevice, s));
-          r2new = r2;
-        }
+      if (was32 && !(r2new < r2)) {  // the cycle made things worse: undo it
+        BDDA_HIP(hipMemcpyAsync(X, ws[0], b64, hipMemcpyDeviceToDevice, s));
+        BDDA_HIP(hipMemcpyAsync(R64, Rsave, b64, hipMemcpyDeviceToDevice, s));
+        r2new = r2;
       }
+      // a further cycle restarts the Krylov space; it runs in complex128
+      conv32 = false;
       r2 = r2new;
       res = std::sqrt(r2) / bnorm;
       if (!hist.empty()) hist.back() = res;

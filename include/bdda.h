@@ -101,9 +101,12 @@ int bdda_block_mvp(bdda_plan* plan, const void* S_dev, void* AS_dev, int L, void
  *                 BDDA_PREC_MIXED: the operator's Goodman convolution in
  *                 complex64 (complex128 scatter / gather boundaries and
  *                 Jacobi epilogue); Krylov blocks, updates, block QR and
- *                 L x L solves in float64; cycles of at most refine_every
- *                 iterations on the float64 true residual, which is
- *                 recomputed with the complex128 operator after each cycle.
+ *                 L x L solves in float64.  Every refine_every iterations
+ *                 (0: never) the complex64 phase continues only while the
+ *                 residual is on a log-linear course to tol by the next
+ *                 check and the complex128 true residual is within 2x of
+ *                 the recursive one; otherwise the solve restarts from its
+ *                 float64 true residual with the complex128 convolution.
  *   B_dev         (3N, L) complex128 right-hand side
  *   X0_dev        (3N, L) complex128 initial guess or NULL (zero start); a
  *                 guess whose preconditioned residual exceeds ||jpre B|| is
