@@ -27,6 +27,9 @@ FLAGS = [
     "-I", os.path.join(ROOT, "include"),
 ]
 SOURCES = ["kernels_fft.hip", "kernels_block.hip", "kernels_fno.hip", "plan.hip"]
+# the FFT kernels fold the additions of their known-zero (pruned) inputs,
+# which IEEE signed zeros would otherwise keep
+FILE_FLAGS = {"kernels_fft.hip": ["-fno-signed-zeros"]}
 # host-only C++ (built with the host compiler)
 CXX_SOURCES = ["host_la.cpp"]
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC"]
@@ -37,7 +40,7 @@ def _compile(src):
     if src.endswith(".cpp"):
         cmd = ["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
     else:
-        cmd = ["hipcc", *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = ["hipcc", *FLAGS, *FILE_FLAGS.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"{cmd[0]} failed for {src}:\n{r.stderr}")

@@ -674,9 +674,202 @@ __global__ void __launch_bounds__(512) k_reg2(LinePass lp) {
   }
 }
 
+// Inverse 16-point DFT, first half of the outputs only:
+// x[n] = sum_k y[k] W16^(-n k) for n < 8, with y[k] = col[k * stride] (LDS).
+// Radix 4 x 4 with k = b + 4 j: for each b, a DFT4 over j gives u_b[kk];
+// x[kk + 4 m] = sum_b W16^(-b kk) u_b[kk] W4^(-b m), and the b sum (m = 0, 1)
+// is accumulated as each group of four inputs arrives, so eight accumulators
+// and one group stay live instead of all sixteen values and their partials.
+template <typename V>
+__device__ __forceinline__ void idft16_lower_half(const V* col, int stride, V* x) {
+  using T = typename RealOf<V>::T;
+  const T c1 = (T)0.92387953251128675613, s1 = (T)0.38268343236508977173, h = (T)0.70710678118654752440;
+  // v * exp(+2 pi i e / 16) for e = b * kk (inverse twiddle)
+  auto tw = [&](V v, int e) -> V {
+    switch (e) {
+      case 1: return cx<V>(v.x * c1 - v.y * s1, v.y * c1 + v.x * s1);
+      case 2: return cx<V>((v.x - v.y) * h, (v.y + v.x) * h);
+      case 3: return cx<V>(v.x * s1 - v.y * c1, v.y * s1 + v.x * c1);
+      case 4: return cx<V>(-v.y, v.x);
+      case 6: return cx<V>((-v.x - v.y) * h, (v.x - v.y) * h);
+      case 9: return cx<V>(-v.x * c1 + v.y * s1, -v.y * c1 - v.x * s1);
+      default: return v;
+    }
+  };
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    V u[4] = {col[b * stride], col[(b + 4) * stride], col[(b + 8) * stride], col[(b + 12) * stride]};
+    Dft<4, true, V>::run(u);
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      const V v = tw(u[kk], b * kk);
+      // W4^(-b) = i^b
+      const V r = b == 0 ? v : b == 1 ? cx<V>(-v.y, v.x) : b == 2 ? cx<V>(-v.x, -v.y) : cx<V>(v.y, -v.x);
+      x[kk] = b == 0 ? v : cadd(x[kk], v);
+      x[kk + 4] = b == 0 ? r : cadd(x[kk + 4], r);
+    }
+  }
+}
+
+// The six Green entries of wavenumber (qx, ky, kz) of the x pass's point pt
+// ((ky, kz), kz fastest), mirrored from the stored half spectrum.
+template <typename V>
+__device__ __forceinline__ void green_row(const LinePass& lp, int pt, int qx, V* g) {
+  const int ky = pt / lp.mz, kz = pt - ky * lp.mz;
+  int ax = qx, ay = ky, az = kz;
+  bool fx = false, fy = false, fz = false;
+  if (ax > lp.mx - ax) { ax = lp.mx - ax; fx = true; }
+  if (ay > lp.my - ay) { ay = lp.my - ay; fy = true; }
+  if (az > lp.mz - az) { az = lp.mz - az; fz = true; }
+  const V* __restrict__ G = reinterpret_cast<const V*>(lp.green) + (((long long)ay * lp.hz + az) * lp.hx + ax) * 6;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) g[c] = G[c];
+  if (fx != fy) g[1] = cx<V>(-g[1].x, -g[1].y);  // xy
+  if (fx != fz) g[2] = cx<V>(-g[2].x, -g[2].y);  // xz
+  if (fy != fz) g[4] = cx<V>(-g[4].x, -g[4].y);  // yz
+}
+
+// ---------------------------------------------------------------------------
+// x pass (forward x FFT, 3x3 Green multiply, inverse x FFT) with one point
+// (ky, kz) per workgroup and a compile-time tile of LB orientations: Q = 3 LB
+// columns, component-major; m = R1 * R2.
+//  forward   thread (q, j2 < R2) holds x[j1 R2 + j2] (j1 < R1): DFT_R1 and
+//            twiddle W^(j2 k1) into LDS row k1 R2 + j2; thread (q, k1 < R1)
+//            reads rows k1 R2 + j2 (j2 < R2): DFT_R2 gives X[k1 + R1 k2],
+//            written back to the rows it read, so row k1 R2 + k2 holds
+//            wavenumber k1 + R1 k2.
+//  multiply  thread (row r, group g) keeps the row's six Green entries in
+//            registers and forms y_c = sum_c' G_cc' x_c' for orientations
+//            g, g + G, ... (G = threads / m groups).
+//  inverse   decimation in time from that digit-reversed order: thread
+//            (q, k1) reads its rows back, inverse DFT_R2 over k2 gives
+//            Z[k1][n2], twiddle W^(-n2 k1), written to the same rows; thread
+//            (q, n2 < R2) reads rows k1 R2 + n2 (k1 < R1): inverse DFT_R1
+//            gives x[n2 + R2 n1].
+// Each thread's step-2 and inverse-step-1 writes go to rows only it reads,
+// so one barrier per exchange suffices (five in all, no reorder pass).  Rows
+// are S = Q | 1 values apart: an odd stride puts the multiply's lanes, which
+// walk rows, on distinct LDS banks.  The launcher guarantees n_in, n_out <=
+// m / 2 (every Goodman line: m >= 2n - 1 and even), so inputs j1 >= R1 / 2
+// are zero and outputs n1 >= R1 / 2 are cropped: they are neither loaded nor
+// stored, the zero operands fold away (this file is built without signed
+// zeros) and the unused outputs are dead code.
+template <typename V, int R1, int R2, int LB>
+__global__ void __launch_bounds__(512) k_xconv(LinePass lp) {
+  constexpr int M = R1 * R2;
+  constexpr int RMAX = R1 > R2 ? R1 : R2;
+  constexpr int Q = 3 * LB;
+  constexpr int S = Q | 1;
+  constexpr int NTHR = Q * RMAX;
+  constexpr int G = NTHR / M;
+  static_assert(NTHR % M == 0 && G >= 1, "the multiply needs whole rows per thread group");
+  static_assert(R1 % 2 == 0, "pruning halves the R1 digit");
+  constexpr int J1 = R1 / 2;  // loaded / stored values of the R1 digit
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  V* buf = reinterpret_cast<V*>(smem_raw);
+  V* tws = buf + M * S;
+  const int tid = threadIdx.x;
+  const int q = tid % Q, w = tid / Q;
+  // 1-D grid remapped so that the column tiles of one point run back to back
+  // on one XCD (blocks are dealt round-robin to the 8 XCDs)
+  const int b = blockIdx.x, xcd = b & 7, slot = b >> 3;
+  const int pt = (slot / lp.nlo) * 8 + xcd;
+  const int lo = slot % lp.nlo;
+  if (pt >= lp.npts) return;
+  const int ga = pt / lp.ptb, gb = pt % lp.ptb;
+  const int se = (int)lp.se, de = (int)lp.de;  // checked at launch
+  const bool s1 = w < R2, s2 = w < R1;
+  V a[RMAX];
+  {
+    const V* __restrict__ base =
+        reinterpret_cast<const V*>(lp.src) + (long long)lo * lp.lo_src + ga * lp.sa + gb * lp.sb + q + w * se;
+    // whole line in range (the common case): plain loads, no per-element
+    // guards to keep alive until the stores
+    if (s1 && (J1 - 1) * R2 + w < lp.nin) {
+#pragma unroll
+      for (int j1 = 0; j1 < R1; ++j1) a[j1] = j1 < J1 ? base[j1 * (R2 * se)] : cx<V>(0, 0);
+    } else {
+#pragma unroll
+      for (int j1 = 0; j1 < R1; ++j1)
+        a[j1] = j1 < J1 ? load_or_zero(s1 && j1 * R2 + w < lp.nin, base + j1 * (R2 * se)) : cx<V>(0, 0);
+    }
+  }
+  {
+    const V twv = load_or_zero(tid < M, reinterpret_cast<const V*>(lp.tw) + tid);
+    if (tid < M) tws[tid] = twv;
+    for (int i = tid + NTHR; i < M; i += NTHR) tws[i] = reinterpret_cast<const V*>(lp.tw)[i];
+  }
+  __syncthreads();  // twiddles staged
+  if (s1) {
+    reg2_step1<R1, R2, false>(a, tws, w);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) buf[(k1 * R2 + w) * S + q] = a[k1];
+  }
+  __syncthreads();
+  if (s2) {
+#pragma unroll
+    for (int j2 = 0; j2 < R2; ++j2) a[j2] = buf[(w * R2 + j2) * S + q];
+    Dft<R2, false, V>::run(a);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) buf[(w * R2 + k2) * S + q] = a[k2];
+  }
+  // Green entries of this thread's multiply row
+  const int r = tid % M, grp = tid / M;
+  V gr[6];
+  green_row<V>(lp, pt, r / R2 + R1 * (r % R2), gr);
+  __syncthreads();
+  {
+    V* row = buf + r * S;
+#pragma unroll
+    for (int li = grp; li < LB; li += G) {
+      const V x0 = row[li], x1 = row[LB + li], x2 = row[2 * LB + li];
+      row[li] = cadd(cadd(cmul(gr[0], x0), cmul(gr[1], x1)), cmul(gr[2], x2));
+      row[LB + li] = cadd(cadd(cmul(gr[1], x0), cmul(gr[3], x1)), cmul(gr[4], x2));
+      row[2 * LB + li] = cadd(cadd(cmul(gr[2], x0), cmul(gr[4], x1)), cmul(gr[5], x2));
+    }
+  }
+  __syncthreads();
+  if (s2) {
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) a[k2] = buf[(w * R2 + k2) * S + q];
+    Dft<R2, true, V>::run(a);
+    if (w != 0) {
+#pragma unroll
+      for (int n2 = 1; n2 < R2; ++n2) {
+        V tw = tws[n2 * w];
+        tw.y = -tw.y;
+        a[n2] = cmul(a[n2], tw);
+      }
+    }
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) buf[(w * R2 + n2) * S + q] = a[n2];
+  }
+  __syncthreads();
+  if (s1) {
+    if constexpr (R1 == 16) {
+      idft16_lower_half<V>(buf + w * S + q, R2 * S, a);
+    } else {
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) a[k1] = buf[(k1 * R2 + w) * S + q];
+      Dft<R1, true, V>::run(a);
+    }
+    V* __restrict__ obase =
+        reinterpret_cast<V*>(lp.dst) + (long long)lo * lp.lo_dst + ga * lp.da + gb * lp.db + q + w * de;
+    if ((J1 - 1) * R2 + w < lp.nout) {
+#pragma unroll
+      for (int n1 = 0; n1 < J1; ++n1) obase[n1 * (R2 * de)] = a[n1];
+    } else {
+#pragma unroll
+      for (int n1 = 0; n1 < J1; ++n1)
+        if (w + R2 * n1 < lp.nout) obase[n1 * (R2 * de)] = a[n1];
+    }
+  }
+}
+
 constexpr int kVPT = 8;
 constexpr int kTile = 4096;  // complex values per workgroup tile
 static std::atomic<bool> g_force_generic{false};
+static std::atomic<bool> g_xconv_off{false};
 
 template <typename V, int MODE>
 static void launch_mode(const LinePass& lp, dim3 grid, int nthr, size_t lds, hipStream_t s) {
@@ -696,6 +889,16 @@ static void launch_reg2_mode(const LinePass& lp, dim3 grid, int nthr, size_t lds
                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
   });
   hipLaunchKernelGGL((k_reg2<V, R1, R2, MODE, LB>), grid, dim3(nthr), lds, s, lp);
+}
+
+template <typename V, int R1, int R2, int LB>
+static void launch_xconv(const LinePass& lp, dim3 grid, int nthr, size_t lds, hipStream_t s) {
+  static std::once_flag attr_once;
+  std::call_once(attr_once, [] {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_xconv<V, R1, R2, LB>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  });
+  hipLaunchKernelGGL((k_xconv<V, R1, R2, LB>), grid, dim3(nthr), lds, s, lp);
 }
 
 template <typename V, int R1, int R2, int LB>
@@ -750,9 +953,19 @@ static void launch_reg2(LinePass lp, hipStream_t s) {
     set_error("register FFT line stride exceeds 32-bit element offsets");
     return;
   }
+  const int nlo = lp.nlo > 0 ? lp.nlo : 1;
+  // x pass on the compiled tile: the dedicated kernel, one point per block
+  if constexpr ((3 * kFixedLb * RMAX) % M == 0 && R1 % 2 == 0) {
+    if (conv && fixed && !g_xconv_off && 2 * lp.nin <= M && 2 * lp.nout <= M) {
+      constexpr int S = (3 * kFixedLb) | 1;
+      const size_t lds_x = ((size_t)M * S + M) * sizeof(V);
+      const dim3 grid_x(((lp.npts + 7) / 8) * 8 * nlo, 1);
+      launch_xconv<V, R1, R2, kFixedLb>(lp, grid_x, 3 * kFixedLb * RMAX, lds_x, s);
+      return;
+    }
+  }
   const size_t lds = ((size_t)M * P * lp.Q + M + (conv ? (size_t)P * M * 6 : 0)) * sizeof(V);
   const int nblk = (lp.npts + P - 1) / P;
-  const int nlo = lp.nlo > 0 ? lp.nlo : 1;
   dim3 grid(nblk, nlo);
   if (conv) grid = dim3(((nblk + 7) / 8) * 8 * nlo, 1);
   if (fixed)
@@ -833,7 +1046,10 @@ void launch_line_pass(const LinePass& lp0, hipStream_t s) {
   }
 }
 
-void set_fft_path(int path) { g_force_generic = path == 1; }
+void set_fft_path(int path) {
+  g_force_generic = path == 1;
+  g_xconv_off = path == 2;
+}
 
 template void launch_line_pass<double2>(const LinePass&, hipStream_t);
 template void launch_line_pass<float2>(const LinePass&, hipStream_t);

@@ -767,8 +767,8 @@ const char* bdda_last_error(void) { return g_err.c_str(); }
 int bdda_version(void) { return 100; }
 
 int bdda_set_fft_path(int path) {
-  if (path != 0 && path != 1) {
-    set_error("bdda_set_fft_path: path must be 0 (auto) or 1 (runtime-radix only)");
+  if (path < 0 || path > 2) {
+    set_error("bdda_set_fft_path: path must be 0 (auto), 1 (runtime-radix only) or 2 (no dedicated x pass)");
     return BDDA_E_ARG;
   }
   set_fft_path(path);

@@ -44,9 +44,12 @@ const char* bdda_last_error(void);
 int bdda_version(void);
 
 /* FFT kernel selection for the whole process: 0 (default) uses the
- * two-step register kernels for lengths m = R1*R2 (R1, R2 in {2,3,4,5,8,16})
- * and the runtime-radix LDS Stockham kernels otherwise; 1 forces the
- * runtime-radix kernels everywhere (used to cross-check the two). */
+ * two-step register kernels for lengths m = R1*R2 (R1, R2 in {2,3,4,5,8,16}),
+ * with the dedicated x-pass kernel where the 8-column tile allows it, and
+ * the runtime-radix LDS Stockham kernels otherwise; 1 forces the
+ * runtime-radix kernels everywhere; 2 is 0 without the dedicated x-pass
+ * kernel (the generic register kernel runs the x pass).  1 and 2 exist to
+ * cross-check the kernels. */
 int bdda_set_fft_path(int path);
 
 /* Plan over an nx x ny x nz cuboid lattice: chooses the convolution FFT

@@ -68,14 +68,45 @@ def test_register_and_stockham_kernels_agree(n):
     At = pm.fft_init(n, 3, lf, k)
     out = {}
     try:
-        for path in (0, 1):
+        for path in (0, 1, 2):
             _lib.check(lib.bdda_set_fft_path(path))
             out[path] = (pm.fft_mvp(n, 3, At, P), pm.fft_mvp(n, 3, At, P, precision="mixed"))
     finally:
         lib.bdda_set_fft_path(0)
-    for path in (0, 1):
+    for path in (0, 1, 2):
         assert rel(out[path][0], ref) < 1e-12
         assert rel(out[path][1], ref) < 2e-6
+
+
+@pytest.mark.parametrize("n", [(8, 5, 4), (12, 5, 4), (16, 5, 4), (24, 5, 4), (32, 4, 3), (64, 6, 5)])
+def test_x_pass_kernel_on_the_compiled_tile(n):
+    """L = 16 takes the compiled 8-column tile, so the x pass runs the
+    dedicated kernel on x lengths 16, 24, 32, 48, 64 and 128 (radix pairs
+    4x4, 8x3, 8x4, 16x3, 8x8, 16x8).  It must agree with the generic register
+    kernel (path 2), the runtime-radix kernels (path 1) and the oracle."""
+    say(f"x pass kernel n={n}")
+    from bdda_amd import _lib
+    from bdda_amd.plan import Plan
+    from mvp_fft import mvp_fft as pm
+    lib = _lib.load()
+    assert Plan(n).column_tile(16) == 8
+    lf, k = 0.031, 8.7
+    rng = np.random.default_rng(11)
+    ncell = int(np.prod(n))
+    P = rng.standard_normal((3 * ncell, 16)) + 1j * rng.standard_normal((3 * ncell, 16))
+    ref = omvp.fft_mvp(n, 3, omvp.fft_init(n, 3, lf, k), P)
+    At = pm.fft_init(n, 3, lf, k)
+    out = {}
+    try:
+        for path in (0, 1, 2):
+            _lib.check(lib.bdda_set_fft_path(path))
+            out[path] = (pm.fft_mvp(n, 3, At, P), pm.fft_mvp(n, 3, At, P, precision="mixed"))
+    finally:
+        lib.bdda_set_fft_path(0)
+    for path in (0, 1, 2):
+        assert rel(out[path][0], ref) < 1e-12, path
+        assert rel(out[path][1], ref) < 2e-6, path
+    assert rel(out[0][0], out[2][0]) < 1e-13
 
 
 def test_column_tile_prefers_the_compiled_8_column_tile():
