@@ -805,6 +805,10 @@ __global__ void __launch_bounds__(512) k_xconv(LinePass lp) {
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) buf[(k1 * R2 + w) * S + q] = a[k1];
   }
+  // Green entries of this thread's multiply row, in flight during step 2
+  const int r = tid % M, grp = tid / M;
+  V gr[6];
+  green_row<V>(lp, pt, r / R2 + R1 * (r % R2), gr);
   __syncthreads();
   if (s2) {
 #pragma unroll
@@ -813,10 +817,6 @@ __global__ void __launch_bounds__(512) k_xconv(LinePass lp) {
 #pragma unroll
     for (int k2 = 0; k2 < R2; ++k2) buf[(w * R2 + k2) * S + q] = a[k2];
   }
-  // Green entries of this thread's multiply row
-  const int r = tid % M, grp = tid / M;
-  V gr[6];
-  green_row<V>(lp, pt, r / R2 + R1 * (r % R2), gr);
   __syncthreads();
   {
     V* row = buf + r * S;
