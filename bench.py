@@ -29,9 +29,10 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def copy_calibration():
+def copy_calibration(timed=20):
     """A plain device copy of known size (2 GiB read, 2 GiB written, three
-    times) for scaling the FETCH_SIZE / WRITE_SIZE counters of rocprofv3."""
+    times) for scaling the FETCH_SIZE / WRITE_SIZE counters of rocprofv3;
+    then `timed` more copies give the rate a plain streaming kernel reaches."""
     import torch
     n = 1 << 28
     x = torch.ones(n, dtype=torch.float64, device="cuda")
@@ -39,7 +40,13 @@ def copy_calibration():
     for _ in range(3):
         y.copy_(x)
     torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes_each": n * 8, "copies": 3}), flush=True)
+    t0 = time.perf_counter()
+    for _ in range(timed):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / timed
+    print(json.dumps({"copy_bytes_each": n * 8, "copies": 3 + timed, "ms_per_copy": round(dt * 1e3, 4),
+                      "copy_GBs_read_plus_write": round(2 * n * 8 / dt / 1e9, 1)}), flush=True)
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "x_conv_f64_gre64_L64_traffic.md")
