@@ -117,6 +117,7 @@ class Plan:
         check(self.lib.bdda_set_target(self.h, ptr(address), N, ptr(posa), ptr(diag), ptr(jpre)), "set_target")
         self.N = N
         self.target_key = key
+        self._jpre_dev = torch.as_tensor(jpre, device="cuda")
 
     def block_mvp(self, S):
         S = to_device(S)
@@ -124,8 +125,13 @@ class Plan:
         check(self.lib.bdda_block_mvp(self.h, ptr(S), ptr(AS), S.shape[1], _stream()), "block_mvp")
         return AS
 
-    def solve(self, B, solver="bicgstab", precision="fp64", tol=1e-2, itermax=25, X0=None, refine_every=20):
-        """Returns (X, iter_fin, err_fin, true_res, history, warm_start_used)."""
+    def solve(self, B, solver="bicgstab", precision="fp64", tol=1e-2, itermax=25, X0=None, refine_every=20,
+              true_residual=True):
+        """Returns (X, iter_fin, err_fin, true_res, history, warm_start_used).
+        A float64 solve stops on its recursive residual, as the reference's
+        solvers do; true_residual=False skips the extra operator application
+        that checks it (true_res is then None).  A mixed solve's err_fin is
+        its true residual already."""
         B = to_device(B)
         L = B.shape[1]
         X = torch.empty_like(B)
@@ -137,11 +143,20 @@ class Plan:
         hist = np.zeros(max(int(itermax), 1), dtype=np.float64)
         check(self.lib.bdda_solve(self.h, _lib.SOLVER[solver], _lib.PRECISION[precision], ptr(B), ptr(X0d),
                                   ptr(X), L, float(tol), int(itermax), int(refine_every), ctypes.byref(it),
-                                  ctypes.byref(err), ctypes.byref(tr),
+                                  ctypes.byref(err), ctypes.byref(tr) if true_residual else None,
                                   hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(flags),
                                   _stream()), "solve")
         nh = flags.value >> 8
-        return X, it.value, err.value, tr.value, hist[:nh].tolist(), bool(flags.value & 1)
+        trv = tr.value if true_residual else (err.value if precision == "mixed" else None)
+        return X, it.value, err.value, trv, hist[:nh].tolist(), bool(flags.value & 1)
+
+    def true_residual(self, B, X):
+        """||jpre (B - A X)||_F / ||jpre B||_F with one complex128 operator
+        application (B, X: (3N, L) complex128 device tensors)."""
+        B = to_device(B)
+        Bj = B * self._jpre_dev[:, None]
+        R = Bj - self.block_mvp(X)
+        return float(torch.linalg.norm(R) / torch.linalg.norm(Bj))
 
     def project_guess(self, B, X0):
         """X0 (3N, L) complex128 device tensor, in place -> X0 C minimising

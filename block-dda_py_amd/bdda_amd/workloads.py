@@ -8,7 +8,7 @@ voxeliser) and the orientations follow the reference's draw order
 import numpy as np
 
 # 64^3 cuboid: GRE bc = 0.8, ab = 0.7, beta = 0.2 (seed 0).  r_v gives a
-# 51 x 48 x 63 lattice (21,113 dipoles) padded with vacuum to 64^3, i.e. a
+# 54 x 51 x 63 lattice (21,434 dipoles) padded with vacuum to 64^3, i.e. a
 # 128^3 doubled FFT grid; birefringent m_p; L = 64; BiCGSTAB to 1e-5.
 FULL_64 = dict(
     name="gre_bc0.8_ab0.7_beta0.2_seed0_64cube_L64",

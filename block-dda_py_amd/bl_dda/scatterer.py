@@ -116,7 +116,7 @@ class DiscreteDipoles(Target, IncidentField):
         self.verbose = DiscreteDipoles.verbose_default
         self.iter_fin = None
         self.err_fin = None
-        self.true_residual = None
+        self._true_residual = None
         self.err_history = []
         self.warm_start_used = False
         self.solve_time = None
@@ -179,8 +179,22 @@ class DiscreteDipoles(Target, IncidentField):
         if self.verbose:
             print("Starting {} iterative solver...".format(self._solver_name()))
         X, it, err, tr, hist, warm = self._plan.solve(self._B_dev, self.solver, self.precision, self.tol,
-                                                      self.itermax, X0=self.X0, refine_every=self.refine_every)
+                                                      self.itermax, X0=self.X0, refine_every=self.refine_every,
+                                                      true_residual=False)
         self._store_solution(X, it, err, tr, hist, warm, time.time() - start)
+
+    @property
+    def true_residual(self):
+        """||jpre (B - A X)|| / ||jpre B|| of the current solution: a mixed
+        solve reports it, a float64 one (which, like the reference, stops on
+        its recursive residual) computes it on first access."""
+        if self._true_residual is None and self._X_dev is not None:
+            self._true_residual = self._plan.true_residual(self._B_dev, self._X_dev)
+        return self._true_residual
+
+    @true_residual.setter
+    def true_residual(self, v):
+        self._true_residual = v
 
     def _solver_name(self):
         return "block-BiCGStab" if self.solver == "bicgstab" else "block-COCG-rq"

@@ -134,66 +134,85 @@ template void launch_gram<float2>(const float2*, const float2*, long long, int, 
                                   hipStream_t);
 
 // ------------------------------------------------------------ block update
-// Y = a X + (W1 + w2c W2) C + e E  over 16-row slabs, output columns tiled by
-// 64.  The L x L coefficients, the combination and the accumulation stay in
-// float64 for both vector types (gfx950 issues float64 FMAs at the float32
-// rate), so a complex64 update is rounded once, at the store: with
-// ill-conditioned coefficients a float accumulation would amplify rounding.
+// Y = a X + (W1 + w2c W2) C + e E.  A block owns a 64-row slab and a 64-column
+// output tile; each of its 256 threads a 4 x 4 register tile (rows ty + 16 i,
+// columns tx + 16 t), so one pass over j reads 4 W and 4 C values from LDS
+// for 16 complex products.  j runs in chunks of 16 (W chunk 64 x 17 with a
+// padded row, C chunk 16 x 64: 33 KB of LDS), in increasing order, so every
+// output is summed in the order j = 0, 1, ..., L-1.  The L x L coefficients,
+// the combination and the accumulation stay in float64 for both vector types
+// (gfx950 issues float64 FMAs at the float32 rate), so a complex64 update is
+// rounded once, at the store: with ill-conditioned coefficients a float
+// accumulation would amplify rounding.  Y may alias X (each output reads its
+// own X entry only), not W1, W2 or E.
+constexpr int kUpdRows = 64, kUpdCols = 64, kUpdChunk = 16;
 template <typename V>
 __global__ void __launch_bounds__(256) k_update(V* Y, const V* X, double2 a, const V* W1, const V* W2,
                                                 double2 w2c, const double2* __restrict__ C, const V* E,
                                                 double2 e, long long rows, int L) {
   using T = typename RealOf<V>::T;
-  extern __shared__ __align__(16) unsigned char sm[];
-  double2* Cs = reinterpret_cast<double2*>(sm);  // [L][64]
-  double2* Ws = Cs + (size_t)L * 64;              // [16][L]
-  const int lt = blockIdx.y;
-  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-  for (int idx = threadIdx.x; idx < L * 64; idx += 256) {
-    const int j = idx / 64, c = idx % 64;
-    const int l = lt * 64 + c;
-    Cs[idx] = l < L ? C[j * L + l] : make_double2(0.0, 0.0);
-  }
-  for (long long r0 = blockIdx.x * 16LL; r0 < rows; r0 += 16LL * gridDim.x) {
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < 16 * L; idx += 256) {
-      const int rr = idx / L, j = idx % L;
-      const long long row = r0 + rr;
-      double2 w = make_double2(0.0, 0.0);
-      if (row < rows) {
-        const V w1 = W1[row * L + j];
-        w = make_double2((double)w1.x, (double)w1.y);
-        if (W2) {
-          const V w2 = W2[row * L + j];
-          w.x += w2c.x * (double)w2.x - w2c.y * (double)w2.y;
-          w.y += w2c.x * (double)w2.y + w2c.y * (double)w2.x;
+  __shared__ double2 Ws[kUpdRows][kUpdChunk + 1];
+  __shared__ double2 Cs[kUpdChunk][kUpdCols];
+  const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
+  const int col0 = blockIdx.y * kUpdCols;
+  for (long long r0 = blockIdx.x * (long long)kUpdRows; r0 < rows; r0 += (long long)kUpdRows * gridDim.x) {
+    double2 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[i][t] = make_double2(0.0, 0.0);
+    for (int j0 = 0; j0 < L; j0 += kUpdChunk) {
+      __syncthreads();  // previous chunk consumed
+#pragma unroll
+      for (int u = 0; u < (kUpdRows * kUpdChunk) / 256; ++u) {
+        const int idx = tid + 256 * u, rr = idx / kUpdChunk, jj = idx % kUpdChunk;
+        const long long row = r0 + rr;
+        const int j = j0 + jj;
+        double2 w = make_double2(0.0, 0.0);
+        if (row < rows && j < L) {
+          const V w1 = W1[row * L + j];
+          w = make_double2((double)w1.x, (double)w1.y);
+          if (W2) {
+            const V w2 = W2[row * L + j];
+            w.x += w2c.x * (double)w2.x - w2c.y * (double)w2.y;
+            w.y += w2c.x * (double)w2.y + w2c.y * (double)w2.x;
+          }
         }
+        Ws[rr][jj] = w;
       }
-      Ws[idx] = w;
-    }
-    __syncthreads();
-    const long long row = r0 + ty;
-    double2 acc[4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = make_double2(0.0, 0.0);
-    const double2* wr = Ws + ty * L;
-    for (int j = 0; j < L; ++j) {
-      const double2 w = wr[j];
-      const double2* cr = Cs + j * 64 + tx * 4;
+      for (int u = 0; u < (kUpdChunk * kUpdCols) / 256; ++u) {
+        const int idx = tid + 256 * u, jj = idx / kUpdCols, cc = idx % kUpdCols;
+        const int j = j0 + jj, l = col0 + cc;
+        Cs[jj][cc] = (j < L && l < L) ? C[(long long)j * L + l] : make_double2(0.0, 0.0);
+      }
+      __syncthreads();
+      const int jn = min(kUpdChunk, L - j0);
+      for (int jj = 0; jj < jn; ++jj) {
+        double2 w[4], c[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) w[i] = Ws[ty + 16 * i][jj];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) c[t] = Cs[jj][tx + 16 * t];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            acc[i][t].x += w[i].x * c[t].x - w[i].y * c[t].y;
+            acc[i][t].y += w[i].x * c[t].y + w[i].y * c[t].x;
+          }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const long long row = r0 + ty + 16 * i;
+      if (row >= rows) continue;
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        const double2 c = cr[t];
-        acc[t].x += w.x * c.x - w.y * c.y;
-        acc[t].y += w.x * c.y + w.y * c.x;
-      }
-    }
-    if (row < rows) {
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int l = lt * 64 + tx * 4 + t;
+        const int l = col0 + tx + 16 * t;
         if (l >= L) continue;
         const long long o = row * L + l;
-        double2 y = acc[t];
+        double2 y = acc[i][t];
         if (X) {
           const V x = X[o];
           y.x += a.x * (double)x.x - a.y * (double)x.y;
@@ -216,16 +235,10 @@ __global__ void __launch_bounds__(256) k_update(V* Y, const V* X, double2 a, con
 template <typename V>
 void launch_update(V* Y, const V* X, double2 a, const V* W1, const V* W2, double2 w2c, const double2* C,
                    const V* E, double2 e, long long rows, int L, hipStream_t s) {
-  const size_t lds = ((size_t)L * 64 + 16 * (size_t)L) * sizeof(double2);
-  static std::once_flag attr_once;
-  std::call_once(attr_once, [] {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_update<V>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-  });
-  long long slabs = (rows + 15) / 16;
-  int g = (int)min(slabs, 4096LL);
-  dim3 grid(g, (L + 63) / 64);
-  hipLaunchKernelGGL((k_update<V>), grid, dim3(256), lds, s, Y, X, a, W1, W2, w2c, C, E, e, rows, L);
+  const long long slabs = (rows + kUpdRows - 1) / kUpdRows;
+  const int g = (int)min(slabs, 4096LL);
+  dim3 grid(g, (L + kUpdCols - 1) / kUpdCols);
+  hipLaunchKernelGGL((k_update<V>), grid, dim3(256), 0, s, Y, X, a, W1, W2, w2c, C, E, e, rows, L);
 }
 template void launch_update<double2>(double2*, const double2*, double2, const double2*, const double2*, double2,
                                      const double2*, const double2*, double2, long long, int, hipStream_t);
