@@ -146,6 +146,40 @@ template <bool INV, typename V> struct Dft<5, INV, V> {
   }
 };
 
+// 12-point DFT as 3 x 4: DFT4 over the stride-3 subsequences x[n1 + 3 n2],
+// twiddles W12^(n1 k2), DFT3 across; X[k2 + 4 k1].
+template <bool INV, typename V> struct Dft<12, INV, V> {
+  __device__ __forceinline__ static void run(V* a) {
+    using T = typename RealOf<V>::T;
+    const T s3 = (T)0.86602540378443864676, h = (T)0.5;
+    V u[3][4];
+#pragma unroll
+    for (int n1 = 0; n1 < 3; ++n1) {
+      V t[4] = {a[n1], a[n1 + 3], a[n1 + 6], a[n1 + 9]};
+      Dft<4, INV, V>::run(t);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) u[n1][k2] = t[k2];
+    }
+    // W12^e = cos(pi e/6) -+ i sin(pi e/6)
+    auto tw = [&](V v, T c, T sn) {  // v * (c - i sn) forward, v * (c + i sn) inverse
+      return INV ? cx<V>(v.x * c - v.y * sn, v.y * c + v.x * sn) : cx<V>(v.x * c + v.y * sn, v.y * c - v.x * sn);
+    };
+    u[1][1] = tw(u[1][1], s3, h);
+    u[1][2] = tw(u[1][2], h, s3);
+    u[1][3] = rotq<INV>(u[1][3]);
+    u[2][1] = tw(u[2][1], h, s3);
+    u[2][2] = tw(u[2][2], -h, s3);
+    u[2][3] = cx<V>(-u[2][3].x, -u[2][3].y);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) {
+      V t[3] = {u[0][k2], u[1][k2], u[2][k2]};
+      Dft<3, INV, V>::run(t);
+#pragma unroll
+      for (int k1 = 0; k1 < 3; ++k1) a[k2 + 4 * k1] = t[k1];
+    }
+  }
+};
+
 // 16-point DFT as 4 x 4: four DFT4 over stride-4 subsequences, twiddles
 // W16^(j2*k1), four DFT4 across; X[k1 + 4 k2].
 template <bool INV, typename V> struct Dft<16, INV, V> {
@@ -415,7 +449,7 @@ __global__ void __launch_bounds__(512) k_line_pass(LinePass lp) {
 }
 
 // ---------------------------------------------------------------------------
-// Two-step register FFT for m = R1 * R2 (R1, R2 in {2, 3, 4, 5, 8, 16}).
+// Two-step register FFT for m = R1 * R2 (R1, R2 in {2, 3, 4, 5, 8, 12, 16}).
 // Step 1: thread (qq, j2) holds x[j1*R2 + j2] (j1 < R1) in registers, does a
 // DFT_R1, multiplies by W_m^(j2*k1) and writes LDS[k1*R2 + j2].  Step 2:
 // thread (qq, k1) reads LDS[k1*R2 + j2] (j2 < R2), does a DFT_R2 and holds
@@ -455,12 +489,17 @@ __device__ __forceinline__ V green_entry(const LinePass& lp, int pt0, int i, int
   return sgn ? cx<V>(-g.x, -g.y) : g;
 }
 
-// Points per workgroup of a register kernel for Q columns per point: as many
-// as 512 threads and a 4096-value tile allow, at least one.
+// Points per workgroup of a register kernel on a compiled tile of Q columns
+// per point: as many as 512 threads and a 4608-value tile (72 KB in
+// complex128) allow, at least one.  Points are consecutive along the
+// innermost axis, so a block's lines are adjacent in memory: at m = 192 on
+// the 4-column tile two points make each element's chunk 384 B instead of
+// 192 B (1.5 cache lines).
+constexpr int kRegTile = 4608;
 constexpr int reg2_points(int M, int RMAX, int Q) {
-  return (512 / (Q * RMAX) < 4096 / (M * Q) ? 512 / (Q * RMAX) : 4096 / (M * Q)) < 1
+  return (512 / (Q * RMAX) < kRegTile / (M * Q) ? 512 / (Q * RMAX) : kRegTile / (M * Q)) < 1
              ? 1
-             : (512 / (Q * RMAX) < 4096 / (M * Q) ? 512 / (Q * RMAX) : 4096 / (M * Q));
+             : (512 / (Q * RMAX) < kRegTile / (M * Q) ? 512 / (Q * RMAX) : kRegTile / (M * Q));
 }
 
 // LB > 0: column tile fixed at LB (Q = 3 LB columns per point, P points per
@@ -740,7 +779,8 @@ __device__ __forceinline__ void green_row(const LinePass& lp, int pt, int qx, V*
 //            wavenumber k1 + R1 k2.
 //  multiply  thread (row r, group g) keeps the row's six Green entries in
 //            registers and forms y_c = sum_c' G_cc' x_c' for orientations
-//            g, g + G, ... (G = threads / m groups).
+//            g, g + G, ... (G = floor(threads / m) groups; the remaining
+//            threads sit this step out).
 //  inverse   decimation in time from that digit-reversed order: thread
 //            (q, k1) reads its rows back, inverse DFT_R2 over k2 gives
 //            Z[k1][n2], twiddle W^(-n2 k1), written to the same rows; thread
@@ -761,8 +801,8 @@ __global__ void __launch_bounds__(512) k_xconv(LinePass lp) {
   constexpr int Q = 3 * LB;
   constexpr int S = Q | 1;
   constexpr int NTHR = Q * RMAX;
-  constexpr int G = NTHR / M;
-  static_assert(NTHR % M == 0 && G >= 1, "the multiply needs whole rows per thread group");
+  constexpr int G = NTHR / M;  // orientation groups of the multiply
+  static_assert(G >= 1, "the multiply needs a thread per row");
   static_assert(R1 % 2 == 0, "pruning halves the R1 digit");
   constexpr int J1 = R1 / 2;  // loaded / stored values of the R1 digit
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -818,7 +858,7 @@ __global__ void __launch_bounds__(512) k_xconv(LinePass lp) {
     for (int k2 = 0; k2 < R2; ++k2) buf[(w * R2 + k2) * S + q] = a[k2];
   }
   __syncthreads();
-  {
+  if (grp < G) {
     V* row = buf + r * S;
 #pragma unroll
     for (int li = grp; li < LB; li += G) {
@@ -926,52 +966,65 @@ static void launch_reg2_tile(const LinePass& lp, dim3 grid, int nthr, size_t lds
   }
 }
 
-// Column tile with a compile-time specialisation (8 columns per component,
-// the tile of L >= 8 on lengths up to 170).
-constexpr int kFixedLb = 8;
+// Column tiles with compile-time specialisations: 8 columns per component
+// (the tile of L >= 8 on lengths up to 170) and 4 (lengths up to 341, e.g.
+// 192 on a 96^3 cuboid).  Tile width, points per block and every LDS offset
+// are then constants; other tiles take them from the launch arguments.
+template <typename V, int R1, int R2, int LB>
+static void launch_reg2_fixed(LinePass lp, hipStream_t s) {
+  constexpr int M = R1 * R2, RMAX = R1 > R2 ? R1 : R2, Q = 3 * LB;
+  const bool conv = lp.mode == PASS_CONV_X;
+  const int nlo = lp.nlo > 0 ? lp.nlo : 1;
+  // x pass: the dedicated kernel, one point per block
+  if constexpr (Q * RMAX >= M && R1 % 2 == 0) {
+    if (conv && !g_xconv_off && 2 * lp.nin <= M && 2 * lp.nout <= M) {
+      constexpr int S = Q | 1;
+      const size_t lds_x = ((size_t)M * S + M) * sizeof(V);
+      const dim3 grid_x(((lp.npts + 7) / 8) * 8 * nlo, 1);
+      launch_xconv<V, R1, R2, LB>(lp, grid_x, Q * RMAX, lds_x, s);
+      return;
+    }
+  }
+  constexpr int P = reg2_points(M, RMAX, Q);
+  lp.P = P;
+  const int nthr = P * Q * RMAX;
+  if (nthr > 512) {
+    set_error("register FFT launch exceeds 512 threads");
+    return;
+  }
+  const size_t lds = ((size_t)M * P * Q + M + (conv ? (size_t)P * M * 6 : 0)) * sizeof(V);
+  const int nblk = (lp.npts + P - 1) / P;
+  dim3 grid(nblk, nlo);
+  if (conv) grid = dim3(((nblk + 7) / 8) * 8 * nlo, 1);
+  launch_reg2_tile<V, R1, R2, LB>(lp, grid, nthr, lds, s);
+}
 
 template <typename V, int R1, int R2>
 static void launch_reg2(LinePass lp, hipStream_t s) {
   constexpr int M = R1 * R2, RMAX = R1 > R2 ? R1 : R2;
-  const bool conv = lp.mode == PASS_CONV_X;
-  const bool fixed = lp.Lb == kFixedLb && lp.Q == 3 * kFixedLb;
-  int P = std::min(512 / (lp.Q * RMAX), 4096 / (M * lp.Q));
-  if (P < 1) P = 1;
-  if (fixed) {
-    P = reg2_points(M, RMAX, 3 * kFixedLb);
-  } else if (P > lp.npts) {
-    P = lp.npts;
+  // element offsets along a line are 32-bit in the kernels
+  if ((long long)M * (lp.se > lp.de ? lp.se : lp.de) >= (1LL << 31)) {
+    set_error("register FFT line stride exceeds 32-bit element offsets");
+    return;
   }
+  if (lp.Q == 3 * lp.Lb && lp.Lb == 8) return launch_reg2_fixed<V, R1, R2, 8>(lp, s);
+  if (lp.Q == 3 * lp.Lb && lp.Lb == 4) return launch_reg2_fixed<V, R1, R2, 4>(lp, s);
+  const bool conv = lp.mode == PASS_CONV_X;
+  int P = std::min(512 / (lp.Q * RMAX), kTile / (M * lp.Q));
+  if (P < 1) P = 1;
+  if (P > lp.npts) P = lp.npts;
   lp.P = P;
   const int nthr = P * lp.Q * RMAX;
   if (nthr > 512) {
     set_error("register FFT launch exceeds 512 threads");
     return;
   }
-  // element offsets along a line are 32-bit in the kernel
-  if ((long long)M * (lp.se > lp.de ? lp.se : lp.de) >= (1LL << 31)) {
-    set_error("register FFT line stride exceeds 32-bit element offsets");
-    return;
-  }
   const int nlo = lp.nlo > 0 ? lp.nlo : 1;
-  // x pass on the compiled tile: the dedicated kernel, one point per block
-  if constexpr ((3 * kFixedLb * RMAX) % M == 0 && R1 % 2 == 0) {
-    if (conv && fixed && !g_xconv_off && 2 * lp.nin <= M && 2 * lp.nout <= M) {
-      constexpr int S = (3 * kFixedLb) | 1;
-      const size_t lds_x = ((size_t)M * S + M) * sizeof(V);
-      const dim3 grid_x(((lp.npts + 7) / 8) * 8 * nlo, 1);
-      launch_xconv<V, R1, R2, kFixedLb>(lp, grid_x, 3 * kFixedLb * RMAX, lds_x, s);
-      return;
-    }
-  }
   const size_t lds = ((size_t)M * P * lp.Q + M + (conv ? (size_t)P * M * 6 : 0)) * sizeof(V);
   const int nblk = (lp.npts + P - 1) / P;
   dim3 grid(nblk, nlo);
   if (conv) grid = dim3(((nblk + 7) / 8) * 8 * nlo, 1);
-  if (fixed)
-    launch_reg2_tile<V, R1, R2, kFixedLb>(lp, grid, nthr, lds, s);
-  else
-    launch_reg2_tile<V, R1, R2, 0>(lp, grid, nthr, lds, s);
+  launch_reg2_tile<V, R1, R2, 0>(lp, grid, nthr, lds, s);
 }
 
 // Lengths with a two-step register kernel (m = R1 * R2); others use the
@@ -980,8 +1033,9 @@ static void launch_reg2(LinePass lp, hipStream_t s) {
 // otherwise the length also goes to the Stockham kernel.
 template <typename V>
 static bool try_reg2(const LinePass& lp, hipStream_t s) {
-  static const int table[][3] = {{16, 4, 4}, {24, 8, 3},  {32, 8, 4},   {40, 8, 5},  {48, 16, 3},
-                                 {64, 8, 8}, {80, 16, 5}, {128, 16, 8}, {256, 16, 16}};
+  static const int table[][3] = {{16, 4, 4},  {24, 8, 3},   {32, 8, 4},    {40, 8, 5},
+                                 {48, 16, 3}, {64, 8, 8},   {80, 16, 5},   {128, 16, 8},
+                                 {192, 16, 12}, {256, 16, 16}};
   for (const auto& t : table) {
     if (t[0] != lp.m) continue;
     const int rmax = t[1] > t[2] ? t[1] : t[2];
@@ -995,6 +1049,7 @@ static bool try_reg2(const LinePass& lp, hipStream_t s) {
       case 64: launch_reg2<V, 8, 8>(lp, s); return true;
       case 80: launch_reg2<V, 16, 5>(lp, s); return true;
       case 128: launch_reg2<V, 16, 8>(lp, s); return true;
+      case 192: launch_reg2<V, 16, 12>(lp, s); return true;
       default: launch_reg2<V, 16, 16>(lp, s); return true;
     }
   }

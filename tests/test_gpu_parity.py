@@ -78,22 +78,26 @@ def test_register_and_stockham_kernels_agree(n):
         assert rel(out[path][1], ref) < 2e-6
 
 
-@pytest.mark.parametrize("n", [(8, 5, 4), (12, 5, 4), (16, 5, 4), (24, 5, 4), (32, 4, 3), (64, 6, 5)])
-def test_x_pass_kernel_on_the_compiled_tile(n):
-    """L = 16 takes the compiled 8-column tile, so the x pass runs the
-    dedicated kernel on x lengths 16, 24, 32, 48, 64 and 128 (radix pairs
-    4x4, 8x3, 8x4, 16x3, 8x8, 16x8).  It must agree with the generic register
-    kernel (path 2), the runtime-radix kernels (path 1) and the oracle."""
-    say(f"x pass kernel n={n}")
+@pytest.mark.parametrize("n,L,lb", [((8, 5, 4), 16, 8), ((12, 5, 4), 16, 8), ((16, 5, 4), 16, 8),
+                                    ((20, 5, 4), 16, 8), ((24, 5, 4), 16, 8), ((32, 4, 3), 16, 8),
+                                    ((40, 5, 4), 16, 8), ((64, 6, 5), 16, 8), ((96, 4, 3), 16, 4),
+                                    ((64, 6, 5), 4, 4), ((16, 5, 4), 4, 4), ((40, 5, 4), 4, 4)])
+def test_x_pass_kernel_on_the_compiled_tile(n, L, lb):
+    """The compiled 8- and 4-column tiles run the x pass in the dedicated
+    kernel, on x lengths 16, 24, 32, 40, 48, 64, 80, 128 and 192 (radix
+    pairs 4x4, 8x3, 8x4, 8x5, 16x3, 8x8, 16x5, 16x8, 16x12).  It must agree
+    with the generic register kernel (path 2), the runtime-radix kernels
+    (path 1) and the oracle."""
+    say(f"x pass kernel n={n} L={L}")
     from bdda_amd import _lib
     from bdda_amd.plan import Plan
     from mvp_fft import mvp_fft as pm
     lib = _lib.load()
-    assert Plan(n).column_tile(16) == 8
+    assert Plan(n).column_tile(L) == lb
     lf, k = 0.031, 8.7
     rng = np.random.default_rng(11)
     ncell = int(np.prod(n))
-    P = rng.standard_normal((3 * ncell, 16)) + 1j * rng.standard_normal((3 * ncell, 16))
+    P = rng.standard_normal((3 * ncell, L)) + 1j * rng.standard_normal((3 * ncell, L))
     ref = omvp.fft_mvp(n, 3, omvp.fft_init(n, 3, lf, k), P)
     At = pm.fft_init(n, 3, lf, k)
     out = {}
@@ -107,6 +111,37 @@ def test_x_pass_kernel_on_the_compiled_tile(n):
         assert rel(out[path][0], ref) < 1e-12, path
         assert rel(out[path][1], ref) < 2e-6, path
     assert rel(out[0][0], out[2][0]) < 1e-13
+
+
+@pytest.mark.parametrize("n,L", [((96, 4, 3), 8), ((4, 96, 3), 8), ((3, 4, 96), 8), ((3, 4, 96), 5)])
+def test_length_192_block_mvp_matches_oracle(n, L):
+    """Length 192 (16 x 12: a 96-cell axis, as on the 96^3 cuboid) on each
+    axis, through the solver's occupied-block path (scatter / gather z
+    passes, Jacobi epilogue), against the oracle's _block_mvp, in both
+    precisions.  L = 8 takes the compiled 4-column tile, L = 5 a runtime one."""
+    say(f"length 192 n={n} L={L}")
+    from bl_krylov import bl_krylov as pk
+    from mvp_fft import mvp_fft as pm
+    from bdda_amd.plan import Plan
+    assert 192 in Plan(n).m
+    lf, k = 0.03, 9.5
+    rng = np.random.default_rng(7)
+    ncell = int(np.prod(n))
+    address = np.sort(rng.choice(ncell, size=ncell // 3, replace=False)).astype(np.int64)
+    N = address.size
+    diag = rng.standard_normal(3 * N) + 1j * rng.standard_normal(3 * N) + 3.0
+    jpre = 1.0 / diag
+    S = rng.standard_normal((3 * N, L)) + 1j * rng.standard_normal((3 * N, L))
+    idx = pk._build_scatter_index(address, 3)
+    a = pk._block_mvp(n, 3, pm.fft_init(n, 3, lf, k), idx, diag, S, jpre)
+    b = okr.block_mvp(n, 3, omvp.fft_init(n, 3, lf, k), idx, diag, S, jpre)
+    assert rel(a, b) < 1e-12
+    # the same operator as the off-diagonal product on the cuboid, mixed
+    P = rng.standard_normal((3 * ncell, L)) + 1j * rng.standard_normal((3 * ncell, L))
+    ref = omvp.fft_mvp(n, 3, omvp.fft_init(n, 3, lf, k), P)
+    At = pm.fft_init(n, 3, lf, k)
+    assert rel(pm.fft_mvp(n, 3, At, P), ref) < 1e-12
+    assert rel(pm.fft_mvp(n, 3, At, P, precision="mixed"), ref) < 2e-6
 
 
 def test_column_tile_prefers_the_compiled_8_column_tile():
