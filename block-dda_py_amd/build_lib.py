@@ -26,10 +26,11 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", os.path.join(ROOT, "include"),
 ]
-SOURCES = ["kernels_fft.hip", "kernels_block.hip", "kernels_fno.hip", "plan.hip"]
+FFT_SOURCES = ["kernels_fft.hip", "fft_len_a.hip", "fft_len_b.hip", "fft_len_c.hip", "fft_len_d.hip"]
+SOURCES = FFT_SOURCES + ["kernels_block.hip", "kernels_fno.hip", "plan.hip"]
 # the FFT kernels fold the additions of their known-zero (pruned) inputs,
 # which IEEE signed zeros would otherwise keep
-FILE_FLAGS = {"kernels_fft.hip": ["-fno-signed-zeros"]}
+FILE_FLAGS = {src: ["-fno-signed-zeros"] for src in FFT_SOURCES}
 # host-only C++ (built with the host compiler)
 CXX_SOURCES = ["host_la.cpp"]
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC"]
@@ -47,7 +48,7 @@ def _compile(src):
     return obj
 
 
-def build(jobs=4, verbose=True):
+def build(jobs=8, verbose=True):
     os.makedirs(OBJ_DIR, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, SOURCES + CXX_SOURCES))
@@ -64,7 +65,7 @@ def build(jobs=4, verbose=True):
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
-    ap.add_argument("--jobs", type=int, default=4)
+    ap.add_argument("--jobs", type=int, default=8)
     a = ap.parse_args()
     try:
         build(a.jobs)

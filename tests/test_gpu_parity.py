@@ -52,10 +52,13 @@ def test_fft_mvp_matches_oracle(n, L):
     assert y1.shape == (3 * ncell,) and rel(y1, ref[:, 0]) < 1e-12
 
 
-@pytest.mark.parametrize("n", [(8, 12, 20), (16, 24, 32), (40, 64, 3), (128, 2, 3)])
+@pytest.mark.parametrize("n", [(8, 12, 20), (16, 24, 32), (40, 64, 3), (128, 2, 3), (10, 15, 18), (23, 25, 27),
+                               (30, 36, 38), (41, 45, 4), (48, 5, 50), (54, 60, 3), (68, 4, 72), (75, 80, 2),
+                               (90, 3, 4), (113, 3, 2), (4, 120, 3), (96, 4, 3)])
 def test_register_and_stockham_kernels_agree(n):
-    """Every register-kernel length (16 ... 256) against the runtime-radix
-    kernel and the oracle, both precisions."""
+    """Every register-kernel length (16 ... 256: radix pairs from
+    {3,4,5,6,8,9,10,12,15,16}) on some axis, at a runtime column tile,
+    against the runtime-radix kernel and the oracle, both precisions."""
     say(f"kernel paths n={n}")
     from bdda_amd import _lib
     from mvp_fft import mvp_fft as pm
@@ -81,12 +84,20 @@ def test_register_and_stockham_kernels_agree(n):
 @pytest.mark.parametrize("n,L,lb", [((8, 5, 4), 16, 8), ((12, 5, 4), 16, 8), ((16, 5, 4), 16, 8),
                                     ((20, 5, 4), 16, 8), ((24, 5, 4), 16, 8), ((32, 4, 3), 16, 8),
                                     ((40, 5, 4), 16, 8), ((64, 6, 5), 16, 8), ((96, 4, 3), 16, 4),
-                                    ((64, 6, 5), 4, 4), ((16, 5, 4), 4, 4), ((40, 5, 4), 4, 4)])
+                                    ((64, 6, 5), 4, 4), ((16, 5, 4), 4, 4), ((40, 5, 4), 4, 4),
+                                    ((10, 5, 4), 16, 8), ((15, 5, 4), 16, 8), ((18, 5, 4), 16, 8),
+                                    ((23, 5, 4), 16, 8), ((25, 5, 4), 16, 8), ((27, 5, 4), 16, 8),
+                                    ((30, 5, 4), 16, 8), ((36, 5, 4), 16, 8), ((38, 5, 4), 16, 8),
+                                    ((41, 5, 4), 16, 8), ((45, 5, 4), 16, 8), ((48, 5, 4), 16, 8),
+                                    ((50, 5, 4), 16, 8), ((54, 5, 4), 16, 8), ((60, 5, 4), 16, 8),
+                                    ((68, 5, 4), 16, 8), ((72, 5, 4), 16, 8), ((75, 5, 4), 16, 8),
+                                    ((80, 5, 4), 16, 8), ((90, 4, 3), 16, 4), ((113, 4, 3), 16, 4),
+                                    ((120, 4, 3), 16, 4), ((54, 5, 4), 4, 4)])
 def test_x_pass_kernel_on_the_compiled_tile(n, L, lb):
     """The compiled 8- and 4-column tiles run the x pass in the dedicated
-    kernel, on x lengths 16, 24, 32, 40, 48, 64, 80, 128 and 192 (radix
-    pairs 4x4, 8x3, 8x4, 8x5, 16x3, 8x8, 16x5, 16x8, 16x12).  It must agree
-    with the generic register kernel (path 2), the runtime-radix kernels
+    kernel wherever R1 is even (x lengths 16 ... 240), and in the generic
+    register kernel for odd R1 (45, 75, 81, 135, 225).  It must agree with
+    the generic register kernel (path 2), the runtime-radix kernels
     (path 1) and the oracle."""
     say(f"x pass kernel n={n} L={L}")
     from bdda_amd import _lib
@@ -113,17 +124,19 @@ def test_x_pass_kernel_on_the_compiled_tile(n, L, lb):
     assert rel(out[0][0], out[2][0]) < 1e-13
 
 
-@pytest.mark.parametrize("n,L", [((96, 4, 3), 8), ((4, 96, 3), 8), ((3, 4, 96), 8), ((3, 4, 96), 5)])
-def test_length_192_block_mvp_matches_oracle(n, L):
+@pytest.mark.parametrize("n,L", [((96, 4, 3), 8), ((4, 96, 3), 8), ((3, 4, 96), 8), ((3, 4, 96), 5)]
+                         + [((3, 4, nz), 8) for nz in (10, 15, 18, 23, 25, 27, 30, 36, 38, 41, 45, 48, 50, 54, 60,
+                                                       68, 72, 75, 80, 90, 113, 120)])
+def test_block_mvp_lengths_match_oracle(n, L):
     """Length 192 (16 x 12: a 96-cell axis, as on the 96^3 cuboid) on each
-    axis, through the solver's occupied-block path (scatter / gather z
-    passes, Jacobi epilogue), against the oracle's _block_mvp, in both
-    precisions.  L = 8 takes the compiled 4-column tile, L = 5 a runtime one."""
-    say(f"length 192 n={n} L={L}")
+    axis, and every other register length on z, through the solver's
+    occupied-block path (scatter / gather z passes, Jacobi epilogue),
+    against the oracle's _block_mvp; then the off-diagonal product on the
+    cuboid in both precisions.  L = 8 takes a compiled tile, L = 5 a runtime
+    one."""
+    say(f"block mvp n={n} L={L}")
     from bl_krylov import bl_krylov as pk
     from mvp_fft import mvp_fft as pm
-    from bdda_amd.plan import Plan
-    assert 192 in Plan(n).m
     lf, k = 0.03, 9.5
     rng = np.random.default_rng(7)
     ncell = int(np.prod(n))
