@@ -590,28 +590,42 @@ __device__ __forceinline__ V green_entry(const LinePass& lp, int pt0, int i, int
 }
 
 // Points per workgroup of a register kernel on a compiled tile of Q columns
-// per point: as many as 512 threads and a 4608-value tile (72 KB in
-// complex128) allow, at least one.  Points are consecutive along the
-// innermost axis, so a block's lines are adjacent in memory: at m = 192 on
-// the 4-column tile two points make each element's chunk 384 B instead of
-// 192 B (1.5 cache lines).
-constexpr int kRegTile = 4608;
+// per point: as many as the thread bound and a 72 KB tile allow, at least
+// one.  Points are consecutive along the innermost axis, so a block's lines
+// are adjacent in memory.  One point's chunk per element is Q*sizeof(V)
+// bytes: 384 B (three cache lines) for a complex128 8-column tile, but
+// 192 B (1.5 lines) for complex64 or a 4-column tile, so those take two
+// points (complex64 blocks on a compiled tile go up to 1024 threads for
+// that; runtime tiles keep 512 and the register budget that comes with it).
+// The z passes' lines are contiguous already (a z line is one run of
+// memory), so they keep 512-thread blocks, which measured faster there.
+template <int MODE>
+constexpr bool is_z_pass() {
+  return is_scatter<MODE>() || is_gather<MODE>() || MODE == PASS_CUBOID_Z || MODE == PASS_CROP_Z;
+}
+template <typename V, int LB, int MODE>
+constexpr int reg2_max_threads() {
+  return sizeof(V) == sizeof(float2) && LB > 0 && !is_z_pass<MODE>() ? 1024 : 512;
+}
+constexpr int kRegTileBytes = 4608 * 16;
+template <typename V, int MODE>
 constexpr int reg2_points(int M, int RMAX, int Q) {
-  return (512 / (Q * RMAX) < kRegTile / (M * Q) ? 512 / (Q * RMAX) : kRegTile / (M * Q)) < 1
-             ? 1
-             : (512 / (Q * RMAX) < kRegTile / (M * Q) ? 512 / (Q * RMAX) : kRegTile / (M * Q));
+  const int by_threads = reg2_max_threads<V, 1, MODE>() / (Q * RMAX);
+  const int by_lds = (kRegTileBytes / (int)sizeof(V)) / (M * Q);
+  const int p = by_threads < by_lds ? by_threads : by_lds;
+  return p < 1 ? 1 : p;
 }
 
 // LB > 0: column tile fixed at LB (Q = 3 LB columns per point, P points per
 // workgroup from reg2_points), so that the tile width and every LDS offset
 // are compile-time constants; LB = 0: tile taken from the launch arguments.
 template <typename V, int R1, int R2, int MODE, int LB>
-__global__ void __launch_bounds__(512) k_reg2(LinePass lp) {
+__global__ void __launch_bounds__((reg2_max_threads<V, LB, MODE>())) k_reg2(LinePass lp) {
   constexpr int M = R1 * R2;
   constexpr int RMAX = R1 > R2 ? R1 : R2;
   constexpr bool INV_PASS = (is_gather<MODE>() || MODE == PASS_CROP_Z || MODE == PASS_PLAIN_INV);
   using BV = typename BoundaryOf<V, MODE>::T;
-  constexpr int PB = LB ? reg2_points(M, RMAX, 3 * LB) : 0;
+  constexpr int PB = LB ? reg2_points<V, MODE>(M, RMAX, 3 * LB) : 0;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int Lb = LB ? LB : lp.Lb;
   const int Q = LB ? 3 * LB : lp.Q;
@@ -1042,29 +1056,52 @@ void launch_xconv(const LinePass& lp, dim3 grid, int nthr, size_t lds, hipStream
   hipLaunchKernelGGL((k_xconv<V, R1, R2, LB>), grid, dim3(nthr), lds, s, lp);
 }
 
-template <typename V, int R1, int R2, int LB>
-void launch_reg2_tile(const LinePass& lp, dim3 grid, int nthr, size_t lds, hipStream_t s) {
+// Calls f(std::integral_constant<int, MODE>{}) for the pass mode of lp.
+template <typename V, typename F>
+void with_reg2_mode(const LinePass& lp, F&& f) {
   const int mode = (lp.mode == PASS_PLAIN && lp.inverse) ? PASS_PLAIN_INV : lp.mode;
   switch (mode) {
-    case PASS_PLAIN: launch_reg2_mode<V, R1, R2, PASS_PLAIN, LB>(lp, grid, nthr, lds, s); break;
-    case PASS_PLAIN_INV: launch_reg2_mode<V, R1, R2, PASS_PLAIN_INV, LB>(lp, grid, nthr, lds, s); break;
-    case PASS_SCATTER_Z: launch_reg2_mode<V, R1, R2, PASS_SCATTER_Z, LB>(lp, grid, nthr, lds, s); break;
-    case PASS_CONV_X: launch_reg2_mode<V, R1, R2, PASS_CONV_X, LB>(lp, grid, nthr, lds, s); break;
-    case PASS_GATHER_Z: launch_reg2_mode<V, R1, R2, PASS_GATHER_Z, LB>(lp, grid, nthr, lds, s); break;
+    case PASS_PLAIN: f(std::integral_constant<int, PASS_PLAIN>{}); break;
+    case PASS_PLAIN_INV: f(std::integral_constant<int, PASS_PLAIN_INV>{}); break;
+    case PASS_SCATTER_Z: f(std::integral_constant<int, PASS_SCATTER_Z>{}); break;
+    case PASS_CONV_X: f(std::integral_constant<int, PASS_CONV_X>{}); break;
+    case PASS_GATHER_Z: f(std::integral_constant<int, PASS_GATHER_Z>{}); break;
     case PASS_SCATTER_Z64:
     case PASS_GATHER_Z64:
       if constexpr (std::is_same<V, float2>::value) {
         if (mode == PASS_SCATTER_Z64)
-          launch_reg2_mode<V, R1, R2, PASS_SCATTER_Z64, LB>(lp, grid, nthr, lds, s);
+          f(std::integral_constant<int, PASS_SCATTER_Z64>{});
         else
-          launch_reg2_mode<V, R1, R2, PASS_GATHER_Z64, LB>(lp, grid, nthr, lds, s);
+          f(std::integral_constant<int, PASS_GATHER_Z64>{});
       } else {
         set_error("register FFT: complex128 boundary modes need a complex64 transform");
       }
       break;
-    case PASS_CUBOID_Z: launch_reg2_mode<V, R1, R2, PASS_CUBOID_Z, LB>(lp, grid, nthr, lds, s); break;
-    default: launch_reg2_mode<V, R1, R2, PASS_CROP_Z, LB>(lp, grid, nthr, lds, s); break;
+    case PASS_CUBOID_Z: f(std::integral_constant<int, PASS_CUBOID_Z>{}); break;
+    default: f(std::integral_constant<int, PASS_CROP_Z>{}); break;
   }
+}
+
+// One pass on a compiled tile: points per block, threads and LDS are
+// compile-time constants of the mode and tile.
+template <typename V, int R1, int R2, int MODE, int LB>
+void launch_reg2_fixed_mode(LinePass lp, hipStream_t s) {
+  constexpr int M = R1 * R2, RMAX = R1 > R2 ? R1 : R2, Q = 3 * LB;
+  constexpr bool conv = MODE == PASS_CONV_X;
+  constexpr int P = reg2_points<V, MODE>(M, RMAX, Q);
+  constexpr int nthr = P * Q * RMAX;
+  static_assert(nthr <= reg2_max_threads<V, LB, MODE>() || P == 1, "tile exceeds the block bound");
+  if (nthr > reg2_max_threads<V, LB, MODE>()) {
+    set_error("register FFT launch exceeds the block's thread bound");
+    return;
+  }
+  lp.P = P;
+  const int nlo = lp.nlo > 0 ? lp.nlo : 1;
+  const size_t lds = ((size_t)M * P * Q + M + (conv ? (size_t)P * M * 6 : 0)) * sizeof(V);
+  const int nblk = (lp.npts + P - 1) / P;
+  dim3 grid(nblk, nlo);
+  if (conv) grid = dim3(((nblk + 7) / 8) * 8 * nlo, 1);
+  launch_reg2_mode<V, R1, R2, MODE, LB>(lp, grid, nthr, lds, s);
 }
 
 // Column tiles with compile-time specialisations: 8 columns per component
@@ -1086,18 +1123,7 @@ void launch_reg2_fixed(LinePass lp, hipStream_t s) {
       return;
     }
   }
-  constexpr int P = reg2_points(M, RMAX, Q);
-  lp.P = P;
-  const int nthr = P * Q * RMAX;
-  if (nthr > 512) {
-    set_error("register FFT launch exceeds 512 threads");
-    return;
-  }
-  const size_t lds = ((size_t)M * P * Q + M + (conv ? (size_t)P * M * 6 : 0)) * sizeof(V);
-  const int nblk = (lp.npts + P - 1) / P;
-  dim3 grid(nblk, nlo);
-  if (conv) grid = dim3(((nblk + 7) / 8) * 8 * nlo, 1);
-  launch_reg2_tile<V, R1, R2, LB>(lp, grid, nthr, lds, s);
+  with_reg2_mode<V>(lp, [&](auto mc) { launch_reg2_fixed_mode<V, R1, R2, decltype(mc)::value, LB>(lp, s); });
 }
 
 template <typename V, int R1, int R2>
@@ -1125,7 +1151,7 @@ void launch_reg2(LinePass lp, hipStream_t s) {
   const int nblk = (lp.npts + P - 1) / P;
   dim3 grid(nblk, nlo);
   if (conv) grid = dim3(((nblk + 7) / 8) * 8 * nlo, 1);
-  launch_reg2_tile<V, R1, R2, 0>(lp, grid, nthr, lds, s);
+  with_reg2_mode<V>(lp, [&](auto mc) { launch_reg2_mode<V, R1, R2, decltype(mc)::value, 0>(lp, grid, nthr, lds, s); });
 }
 
 // Register-kernel lengths m = R1 * R2, in four groups (one per translation
