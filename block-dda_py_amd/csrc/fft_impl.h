@@ -908,52 +908,43 @@ __device__ __forceinline__ void green_row(const LinePass& lp, int pt, int qx, V*
 // are zero and outputs n1 >= R1 / 2 are cropped: they are neither loaded nor
 // stored, the zero operands fold away (this file is built without signed
 // zeros) and the unused outputs are dead code.
+// Inputs of one x line: thread (q, w < R2) loads x[j1 R2 + w] for the
+// j1 < R1 / 2 that can be nonzero (zero beyond n_in).
 template <typename V, int R1, int R2, int LB>
-__global__ void __launch_bounds__(512) k_xconv(LinePass lp) {
+__device__ __forceinline__ void xconv_load(const LinePass& lp, int pt, int lo, int q, int w, V* an) {
+  constexpr int J1 = R1 / 2;
+  const int ga = pt / lp.ptb, gb = pt % lp.ptb;
+  const int se = (int)lp.se;  // checked at launch
+  const V* __restrict__ base =
+      reinterpret_cast<const V*>(lp.src) + (long long)lo * lp.lo_src + ga * lp.sa + gb * lp.sb + q + w * se;
+  // whole line in range (the common case): plain loads, no per-element
+  // guards to keep alive until the stores
+  if (w < R2 && (J1 - 1) * R2 + w < lp.nin) {
+#pragma unroll
+    for (int j1 = 0; j1 < J1; ++j1) an[j1] = base[j1 * (R2 * se)];
+  } else {
+#pragma unroll
+    for (int j1 = 0; j1 < J1; ++j1) an[j1] = load_or_zero(w < R2 && j1 * R2 + w < lp.nin, base + j1 * (R2 * se));
+  }
+}
+
+// One x line tile after its inputs are in a[0, R1/2): forward transform, 3x3
+// Green multiply, inverse transform, store.  Needs the twiddles staged in
+// tws; buf is free again only after a barrier.
+template <typename V, int R1, int R2, int LB>
+__device__ __forceinline__ void xconv_point(const LinePass& lp, V* buf, const V* tws, V* a, int pt, int lo,
+                                            int tid) {
   constexpr int M = R1 * R2;
   constexpr int RMAX = R1 > R2 ? R1 : R2;
   constexpr int Q = 3 * LB;
   constexpr int S = Q | 1;
   constexpr int NTHR = Q * RMAX;
   constexpr int G = NTHR / M;  // orientation groups of the multiply
-  static_assert(G >= 1, "the multiply needs a thread per row");
-  static_assert(R1 % 2 == 0, "pruning halves the R1 digit");
-  constexpr int J1 = R1 / 2;  // loaded / stored values of the R1 digit
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  V* buf = reinterpret_cast<V*>(smem_raw);
-  V* tws = buf + M * S;
-  const int tid = threadIdx.x;
+  constexpr int J1 = R1 / 2;
   const int q = tid % Q, w = tid / Q;
-  // 1-D grid remapped so that the column tiles of one point run back to back
-  // on one XCD (blocks are dealt round-robin to the 8 XCDs)
-  const int b = blockIdx.x, xcd = b & 7, slot = b >> 3;
-  const int pt = (slot / lp.nlo) * 8 + xcd;
-  const int lo = slot % lp.nlo;
-  if (pt >= lp.npts) return;
-  const int ga = pt / lp.ptb, gb = pt % lp.ptb;
-  const int se = (int)lp.se, de = (int)lp.de;  // checked at launch
   const bool s1 = w < R2, s2 = w < R1;
-  V a[RMAX];
-  {
-    const V* __restrict__ base =
-        reinterpret_cast<const V*>(lp.src) + (long long)lo * lp.lo_src + ga * lp.sa + gb * lp.sb + q + w * se;
-    // whole line in range (the common case): plain loads, no per-element
-    // guards to keep alive until the stores
-    if (s1 && (J1 - 1) * R2 + w < lp.nin) {
 #pragma unroll
-      for (int j1 = 0; j1 < R1; ++j1) a[j1] = j1 < J1 ? base[j1 * (R2 * se)] : cx<V>(0, 0);
-    } else {
-#pragma unroll
-      for (int j1 = 0; j1 < R1; ++j1)
-        a[j1] = j1 < J1 ? load_or_zero(s1 && j1 * R2 + w < lp.nin, base + j1 * (R2 * se)) : cx<V>(0, 0);
-    }
-  }
-  {
-    const V twv = load_or_zero(tid < M, reinterpret_cast<const V*>(lp.tw) + tid);
-    if (tid < M) tws[tid] = twv;
-    for (int i = tid + NTHR; i < M; i += NTHR) tws[i] = reinterpret_cast<const V*>(lp.tw)[i];
-  }
-  __syncthreads();  // twiddles staged
+  for (int j1 = J1; j1 < R1; ++j1) a[j1] = cx<V>(0, 0);
   if (s1) {
     reg2_step1<R1, R2, false>(a, tws, w);
 #pragma unroll
@@ -1007,6 +998,8 @@ __global__ void __launch_bounds__(512) k_xconv(LinePass lp) {
       for (int k1 = 0; k1 < R1; ++k1) a[k1] = buf[(k1 * R2 + w) * S + q];
       Dft<R1, true, V>::run(a);
     }
+    const int ga = pt / lp.ptb, gb = pt % lp.ptb;
+    const int de = (int)lp.de;  // checked at launch
     V* __restrict__ obase =
         reinterpret_cast<V*>(lp.dst) + (long long)lo * lp.lo_dst + ga * lp.da + gb * lp.db + q + w * de;
     if ((J1 - 1) * R2 + w < lp.nout) {
@@ -1018,6 +1011,43 @@ __global__ void __launch_bounds__(512) k_xconv(LinePass lp) {
         if (w + R2 * n1 < lp.nout) obase[n1 * (R2 * de)] = a[n1];
     }
   }
+}
+
+// Block i of the x pass -> (point, column tile): blocks are dealt to the 8
+// XCDs round-robin (i % 8 is the XCD), and the column tiles of one point are
+// consecutive blocks on one XCD, where that point's Green rows then stay in
+// L2.
+__device__ __forceinline__ void xconv_item(const LinePass& lp, int item, int* pt, int* lo) {
+  const int xcd = item & 7, slot = item >> 3;
+  *pt = (slot / lp.nlo) * 8 + xcd;
+  *lo = slot % lp.nlo;
+}
+
+template <typename V, int R1, int R2, int LB>
+__global__ void __launch_bounds__(512) k_xconv(LinePass lp) {
+  constexpr int M = R1 * R2;
+  constexpr int RMAX = R1 > R2 ? R1 : R2;
+  constexpr int Q = 3 * LB;
+  constexpr int S = Q | 1;
+  constexpr int NTHR = Q * RMAX;
+  static_assert(NTHR / M >= 1, "the multiply needs a thread per row");
+  static_assert(R1 % 2 == 0, "pruning halves the R1 digit");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  V* buf = reinterpret_cast<V*>(smem_raw);
+  V* tws = buf + M * S;
+  const int tid = threadIdx.x;
+  int pt, lo;
+  xconv_item(lp, blockIdx.x, &pt, &lo);
+  if (pt >= lp.npts) return;
+  V a[RMAX];
+  xconv_load<V, R1, R2, LB>(lp, pt, lo, tid % Q, tid / Q, a);
+  {
+    const V twv = load_or_zero(tid < M, reinterpret_cast<const V*>(lp.tw) + tid);
+    if (tid < M) tws[tid] = twv;
+    for (int i = tid + NTHR; i < M; i += NTHR) tws[i] = reinterpret_cast<const V*>(lp.tw)[i];
+  }
+  __syncthreads();  // twiddles staged
+  xconv_point<V, R1, R2, LB>(lp, buf, tws, a, pt, lo, tid);
 }
 
 constexpr int kVPT = 8;

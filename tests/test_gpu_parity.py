@@ -98,7 +98,7 @@ def test_x_pass_kernel_on_the_compiled_tile(n, L, lb):
     kernel wherever R1 is even (x lengths 16 ... 240), and in the generic
     register kernel for odd R1 (45, 75, 81, 135, 225).  It must agree with
     the generic register kernel (path 2), the runtime-radix kernels
-    (path 1) and the oracle."""
+    (path 1) and the oracle, and repeat bit for bit."""
     say(f"x pass kernel n={n} L={L}")
     from bdda_amd import _lib
     from bdda_amd.plan import Plan
@@ -116,12 +116,16 @@ def test_x_pass_kernel_on_the_compiled_tile(n, L, lb):
         for path in (0, 1, 2):
             _lib.check(lib.bdda_set_fft_path(path))
             out[path] = (pm.fft_mvp(n, 3, At, P), pm.fft_mvp(n, 3, At, P, precision="mixed"))
+        _lib.check(lib.bdda_set_fft_path(0))
+        again = (pm.fft_mvp(n, 3, At, P), pm.fft_mvp(n, 3, At, P, precision="mixed"))
     finally:
         lib.bdda_set_fft_path(0)
     for path in (0, 1, 2):
         assert rel(out[path][0], ref) < 1e-12, path
         assert rel(out[path][1], ref) < 2e-6, path
     assert rel(out[0][0], out[2][0]) < 1e-13
+    # the dedicated kernel repeats bit for bit
+    assert np.array_equal(out[0][0], again[0]) and np.array_equal(out[0][1], again[1])
 
 
 @pytest.mark.parametrize("n,L", [((96, 4, 3), 8), ((4, 96, 3), 8), ((3, 4, 96), 8), ((3, 4, 96), 5)]
