@@ -34,12 +34,18 @@ __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
 constexpr int kGramParts = 512;
 int gram_parts() { return kGramParts; }
 
+// A block owns a contiguous range of rows and one 64 x 64 output tile; each
+// of its 256 threads a 4 x 4 register tile (rows ty + 16 i, columns
+// tx + 16 j: lanes walk consecutive LDS words, so reads are conflict-free).
+// Rows go through LDS in slabs of 16; the next slab is loaded into
+// registers while the current one is multiplied.  Every output sums its
+// rows in increasing order.
 template <typename V, bool CONJ>
 __global__ void __launch_bounds__(256) k_gram(const V* __restrict__ A, const V* __restrict__ B, long long rows,
                                               int L, double2* __restrict__ partial) {
   __shared__ double2 As[16][64];
   __shared__ double2 Bs[16][64];
-  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
   const int ntile = (L + 63) / 64;
   const int jt = blockIdx.y / ntile, lt = blockIdx.y % ntile;
   const long long chunk = (rows + gridDim.x - 1) / gridDim.x;
@@ -50,28 +56,37 @@ __global__ void __launch_bounds__(256) k_gram(const V* __restrict__ A, const V* 
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = make_double2(0.0, 0.0);
-  for (long long rs = r0; rs < r1; rs += 16) {
-    for (int idx = threadIdx.x; idx < 16 * 64; idx += 256) {
-      const int rr = idx / 64, cc = idx % 64;
+  // this thread's share of a slab: entries tid + 256 u (u < 4) of [16][64]
+  double2 pa[4], pb[4];
+  auto fetch = [&](long long rs) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int idx = tid + 256 * u, rr = idx / 64, cc = idx % 64;
       const long long row = rs + rr;
       const int ja = jt * 64 + cc, lb = lt * 64 + cc;
-      double2 va = make_double2(0.0, 0.0), vb = make_double2(0.0, 0.0);
-      if (row < r1) {
-        if (ja < L) va = d2(A[row * L + ja]);
-        if (lb < L) vb = d2(B[row * L + lb]);
-      }
-      if (CONJ) va.y = -va.y;
-      As[rr][cc] = va;
-      Bs[rr][cc] = vb;
+      pa[u] = (row < r1 && ja < L) ? d2(A[row * L + ja]) : make_double2(0.0, 0.0);
+      pb[u] = (row < r1 && lb < L) ? d2(B[row * L + lb]) : make_double2(0.0, 0.0);
+      if (CONJ) pa[u].y = -pa[u].y;
+    }
+  };
+  if (r0 < r1) fetch(r0);
+  for (long long rs = r0; rs < r1; rs += 16) {
+    __syncthreads();  // the previous slab has been consumed
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int idx = tid + 256 * u;
+      As[idx / 64][idx % 64] = pa[u];
+      Bs[idx / 64][idx % 64] = pb[u];
     }
     __syncthreads();
+    if (rs + 16 < r1) fetch(rs + 16);  // in flight during the products
 #pragma unroll 4
     for (int rr = 0; rr < 16; ++rr) {
       double2 a[4], b[4];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        a[t] = As[rr][ty * 4 + t];
-        b[t] = Bs[rr][tx * 4 + t];
+        a[t] = As[rr][ty + 16 * t];
+        b[t] = Bs[rr][tx + 16 * t];
       }
 #pragma unroll
       for (int i = 0; i < 4; ++i)
@@ -81,13 +96,12 @@ __global__ void __launch_bounds__(256) k_gram(const V* __restrict__ A, const V* 
           acc[i][j].y += a[i].x * b[j].y + a[i].y * b[j].x;
         }
     }
-    __syncthreads();
   }
   double2* out = partial + ((long long)blockIdx.x * gridDim.y + blockIdx.y) * 4096;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) out[(ty * 4 + i) * 64 + tx * 4 + j] = acc[i][j];
+    for (int j = 0; j < 4; ++j) out[(ty + 16 * i) * 64 + tx + 16 * j] = acc[i][j];
 }
 
 // out[j][l] = sum of the parts' 64 x 64 tiles.  A block takes 16 consecutive
