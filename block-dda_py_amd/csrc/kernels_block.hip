@@ -34,19 +34,23 @@ __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
 constexpr int kGramParts = 512;
 int gram_parts() { return kGramParts; }
 
-// A block owns a contiguous range of rows and one 64 x 64 output tile; each
-// of its 256 threads a 4 x 4 register tile (rows ty + 16 i, columns
-// tx + 16 j: lanes walk consecutive LDS words, so reads are conflict-free).
-// Rows go through LDS in slabs of 16; the next slab is loaded into
-// registers while the current one is multiplied.  Every output sums its
-// rows in increasing order.
-template <typename V, bool CONJ>
+// A block owns a contiguous range of rows and one TL x TL output tile
+// (TL = 64, 32 or 16: the smallest that holds L, so that small L do not
+// multiply padding).  Its 256 threads form NG = 256 / (TL/4)^2 groups, each
+// with a 4 x 4 register tile per thread (rows ty + TL/4 i, columns
+// tx + TL/4 j: lanes walk consecutive LDS words, so reads are
+// conflict-free); group g takes the rows g, g + NG, ... of every 16-row slab
+// and writes its own partial tile.  The next slab is loaded into registers
+// while the current one is multiplied.
+template <typename V, bool CONJ, int TL>
 __global__ void __launch_bounds__(256) k_gram(const V* __restrict__ A, const V* __restrict__ B, long long rows,
                                               int L, double2* __restrict__ partial) {
-  __shared__ double2 As[16][64];
-  __shared__ double2 Bs[16][64];
-  const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
-  const int ntile = (L + 63) / 64;
+  constexpr int TQ = TL / 4, TPG = TQ * TQ, NG = 256 / TPG;
+  constexpr int PER = (16 * TL) / 256;  // slab entries per thread
+  __shared__ double2 As[16][TL];
+  __shared__ double2 Bs[16][TL];
+  const int tid = threadIdx.x, g = tid / TPG, t = tid % TPG, tx = t % TQ, ty = t / TQ;
+  const int ntile = (L + TL - 1) / TL;
   const int jt = blockIdx.y / ntile, lt = blockIdx.y % ntile;
   const long long chunk = (rows + gridDim.x - 1) / gridDim.x;
   const long long r0 = blockIdx.x * chunk;
@@ -56,14 +60,14 @@ __global__ void __launch_bounds__(256) k_gram(const V* __restrict__ A, const V* 
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = make_double2(0.0, 0.0);
-  // this thread's share of a slab: entries tid + 256 u (u < 4) of [16][64]
-  double2 pa[4], pb[4];
+  // this thread's share of a slab: entries tid + 256 u (u < PER) of [16][TL]
+  double2 pa[PER], pb[PER];
   auto fetch = [&](long long rs) {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int idx = tid + 256 * u, rr = idx / 64, cc = idx % 64;
+    for (int u = 0; u < PER; ++u) {
+      const int idx = tid + 256 * u, rr = idx / TL, cc = idx % TL;
       const long long row = rs + rr;
-      const int ja = jt * 64 + cc, lb = lt * 64 + cc;
+      const int ja = jt * TL + cc, lb = lt * TL + cc;
       pa[u] = (row < r1 && ja < L) ? d2(A[row * L + ja]) : make_double2(0.0, 0.0);
       pb[u] = (row < r1 && lb < L) ? d2(B[row * L + lb]) : make_double2(0.0, 0.0);
       if (CONJ) pa[u].y = -pa[u].y;
@@ -73,20 +77,20 @@ __global__ void __launch_bounds__(256) k_gram(const V* __restrict__ A, const V* 
   for (long long rs = r0; rs < r1; rs += 16) {
     __syncthreads();  // the previous slab has been consumed
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < PER; ++u) {
       const int idx = tid + 256 * u;
-      As[idx / 64][idx % 64] = pa[u];
-      Bs[idx / 64][idx % 64] = pb[u];
+      As[idx / TL][idx % TL] = pa[u];
+      Bs[idx / TL][idx % TL] = pb[u];
     }
     __syncthreads();
     if (rs + 16 < r1) fetch(rs + 16);  // in flight during the products
-#pragma unroll 4
-    for (int rr = 0; rr < 16; ++rr) {
+#pragma unroll
+    for (int rr = g; rr < 16; rr += NG) {
       double2 a[4], b[4];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        a[t] = As[rr][ty + 16 * t];
-        b[t] = Bs[rr][tx + 16 * t];
+      for (int q = 0; q < 4; ++q) {
+        a[q] = As[rr][ty + TQ * q];
+        b[q] = Bs[rr][tx + TQ * q];
       }
 #pragma unroll
       for (int i = 0; i < 4; ++i)
@@ -97,30 +101,30 @@ __global__ void __launch_bounds__(256) k_gram(const V* __restrict__ A, const V* 
         }
     }
   }
-  double2* out = partial + ((long long)blockIdx.x * gridDim.y + blockIdx.y) * 4096;
+  double2* out = partial + (((long long)blockIdx.x * NG + g) * gridDim.y + blockIdx.y) * (TL * TL);
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) out[(ty + 16 * i) * 64 + tx + 16 * j] = acc[i][j];
+    for (int j = 0; j < 4; ++j) out[(ty + TQ * i) * TL + tx + TQ * j] = acc[i][j];
 }
 
-// out[j][l] = sum of the parts' 64 x 64 tiles.  A block takes 16 consecutive
-// tile entries and 16 part groups: group g sums parts g, g + 16, ..., and the
-// 16 group sums are added in group order (a fixed order: deterministic).
-// Entries outside the L x L result are skipped.
+// out[j][l] = sum of the parts' TL x TL tiles.  A block takes 16
+// consecutive tile entries and 16 part groups: group g sums parts g, g + 16,
+// ..., and the 16 group sums are added in group order (a fixed order:
+// deterministic).  Entries outside the L x L result are skipped.
 constexpr int kRedEntries = 16, kRedGroups = 16;
-__global__ void __launch_bounds__(256) k_gram_reduce(const double2* __restrict__ partial, int nparts, int ntile2,
-                                                     int L, double2* __restrict__ out) {
+__global__ void __launch_bounds__(256) k_gram_reduce(const double2* __restrict__ partial, int nparts, int ntile,
+                                                     int TL, int L, double2* __restrict__ out) {
   __shared__ double2 sh[kRedGroups][kRedEntries];
   const int e = threadIdx.x % kRedEntries, g = threadIdx.x / kRedEntries;
   const long long id = (long long)blockIdx.x * kRedEntries + e;
-  const int ntile = (L + 63) / 64;
-  const int tile = (int)(id / 4096), w = (int)(id % 4096);
-  const int j = (tile / ntile) * 64 + w / 64, l = (tile % ntile) * 64 + w % 64;
-  const bool live = id < (long long)ntile2 * 4096 && j < L && l < L;
+  const int te = TL * TL, ntile2 = ntile * ntile;
+  const int tile = (int)(id / te), w = (int)(id % te);
+  const int j = (tile / ntile) * TL + w / TL, l = (tile % ntile) * TL + w % TL;
+  const bool live = id < (long long)ntile2 * te && j < L && l < L;
   double2 s = make_double2(0.0, 0.0);
   if (live)
-    for (int p = g; p < nparts; p += kRedGroups) s = zadd(s, partial[((long long)p * ntile2 + tile) * 4096 + w]);
+    for (int p = g; p < nparts; p += kRedGroups) s = zadd(s, partial[((long long)p * ntile2 + tile) * te + w]);
   sh[g][e] = s;
   __syncthreads();
   if (g == 0 && live) {
@@ -129,18 +133,35 @@ __global__ void __launch_bounds__(256) k_gram_reduce(const double2* __restrict__
   }
 }
 
+// Output tile of the Gram and update kernels: the smallest of 16, 32, 64
+// that holds L (64 beyond).
+static int block_tile(int L) { return L <= 16 ? 16 : (L <= 32 ? 32 : 64); }
+
+template <typename V, int TL>
+static void launch_gram_tile(const V* A, const V* B, long long rows, int L, bool conjA, double2* partial,
+                             int nparts, double2* out, hipStream_t s) {
+  constexpr int NG = 256 / ((TL / 4) * (TL / 4));
+  const int ntile = (L + TL - 1) / TL;
+  dim3 grid(nparts, ntile * ntile);
+  if (conjA)
+    hipLaunchKernelGGL((k_gram<V, true, TL>), grid, dim3(256), 0, s, A, B, rows, L, partial);
+  else
+    hipLaunchKernelGGL((k_gram<V, false, TL>), grid, dim3(256), 0, s, A, B, rows, L, partial);
+  const long long entries = (long long)ntile * ntile * TL * TL;
+  hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)((entries + kRedEntries - 1) / kRedEntries)),
+                     dim3(kRedEntries * kRedGroups), 0, s, partial, nparts * NG, ntile, TL, L, out);
+}
+
+// The partial buffer holds nparts * 4096 * ceil(L / 64)^2 entries, which
+// covers every tile: (nparts * NG) * TL^2 = nparts * 4096 for TL < 64.
 template <typename V>
 void launch_gram(const V* A, const V* B, long long rows, int L, bool conjA, double2* partial, int nparts,
                  double2* out, hipStream_t s) {
-  const int ntile = (L + 63) / 64;
-  dim3 grid(nparts, ntile * ntile);
-  if (conjA)
-    hipLaunchKernelGGL((k_gram<V, true>), grid, dim3(256), 0, s, A, B, rows, L, partial);
-  else
-    hipLaunchKernelGGL((k_gram<V, false>), grid, dim3(256), 0, s, A, B, rows, L, partial);
-  const long long entries = (long long)ntile * ntile * 4096;
-  hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)((entries + kRedEntries - 1) / kRedEntries)),
-                     dim3(kRedEntries * kRedGroups), 0, s, partial, nparts, ntile * ntile, L, out);
+  switch (block_tile(L)) {
+    case 16: launch_gram_tile<V, 16>(A, B, rows, L, conjA, partial, nparts, out, s); break;
+    case 32: launch_gram_tile<V, 32>(A, B, rows, L, conjA, partial, nparts, out, s); break;
+    default: launch_gram_tile<V, 64>(A, B, rows, L, conjA, partial, nparts, out, s); break;
+  }
 }
 template void launch_gram<double2>(const double2*, const double2*, long long, int, bool, double2*, int, double2*,
                                    hipStream_t);
@@ -148,38 +169,39 @@ template void launch_gram<float2>(const float2*, const float2*, long long, int, 
                                   hipStream_t);
 
 // ------------------------------------------------------------ block update
-// Y = a X + (W1 + w2c W2) C + e E.  A block owns a 64-row slab and a 64-column
-// output tile; each of its 256 threads a 4 x 4 register tile (rows ty + 16 i,
-// columns tx + 16 t), so one pass over j reads 4 W and 4 C values from LDS
-// for 16 complex products.  j runs in chunks of 16 (W chunk 64 x 17 with a
-// padded row, C chunk 16 x 64: 33 KB of LDS), in increasing order, so every
-// output is summed in the order j = 0, 1, ..., L-1.  The L x L coefficients,
-// the combination and the accumulation stay in float64 for both vector types
-// (gfx950 issues float64 FMAs at the float32 rate), so a complex64 update is
-// rounded once, at the store: with ill-conditioned coefficients a float
-// accumulation would amplify rounding.  Y may alias X (each output reads its
-// own X entry only), not W1, W2 or E.
-constexpr int kUpdRows = 64, kUpdCols = 64, kUpdChunk = 16;
-template <typename V>
+// Y = a X + (W1 + w2c W2) C + e E.  A block owns a slab of RB = 4096 / TL
+// rows and a TL-column output tile (TL = 64, 32 or 16, the smallest that
+// holds L, so that small L do not multiply padding); each of its 256 threads
+// a 4 x 4 register tile (rows ty + RB/4 i, columns tx + TL/4 t), so one pass
+// over j reads 4 W and 4 C values from LDS for 16 complex products.  j runs
+// in chunks of KC (W chunk RB x KC+1 with a padded row, C chunk KC x TL), in
+// increasing order, so every output is summed in the order j = 0, ..., L-1.
+// The L x L coefficients, the combination and the accumulation stay in
+// float64 for both vector types (gfx950 issues float64 FMAs at the float32
+// rate), so a complex64 update is rounded once, at the store: with
+// ill-conditioned coefficients a float accumulation would amplify rounding.
+// Y may alias X (each output reads its own X entry only), not W1, W2 or E.
+template <typename V, int TL>
 __global__ void __launch_bounds__(256) k_update(V* Y, const V* X, double2 a, const V* W1, const V* W2,
                                                 double2 w2c, const double2* __restrict__ C, const V* E,
                                                 double2 e, long long rows, int L) {
   using T = typename RealOf<V>::T;
-  __shared__ double2 Ws[kUpdRows][kUpdChunk + 1];
-  __shared__ double2 Cs[kUpdChunk][kUpdCols];
-  const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
-  const int col0 = blockIdx.y * kUpdCols;
-  for (long long r0 = blockIdx.x * (long long)kUpdRows; r0 < rows; r0 += (long long)kUpdRows * gridDim.x) {
+  constexpr int TQ = TL / 4, RB = 4096 / TL, RQ = RB / 4, KC = TL == 16 ? 8 : 16;
+  __shared__ double2 Ws[RB][KC + 1];
+  __shared__ double2 Cs[KC][TL];
+  const int tid = threadIdx.x, tx = tid % TQ, ty = tid / TQ;
+  const int col0 = blockIdx.y * TL;
+  for (long long r0 = blockIdx.x * (long long)RB; r0 < rows; r0 += (long long)RB * gridDim.x) {
     double2 acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int t = 0; t < 4; ++t) acc[i][t] = make_double2(0.0, 0.0);
-    for (int j0 = 0; j0 < L; j0 += kUpdChunk) {
+    for (int j0 = 0; j0 < L; j0 += KC) {
       __syncthreads();  // previous chunk consumed
 #pragma unroll
-      for (int u = 0; u < (kUpdRows * kUpdChunk) / 256; ++u) {
-        const int idx = tid + 256 * u, rr = idx / kUpdChunk, jj = idx % kUpdChunk;
+      for (int u = 0; u < (RB * KC) / 256; ++u) {
+        const int idx = tid + 256 * u, rr = idx / KC, jj = idx % KC;
         const long long row = r0 + rr;
         const int j = j0 + jj;
         double2 w = make_double2(0.0, 0.0);
@@ -194,20 +216,19 @@ __global__ void __launch_bounds__(256) k_update(V* Y, const V* X, double2 a, con
         }
         Ws[rr][jj] = w;
       }
-#pragma unroll
-      for (int u = 0; u < (kUpdChunk * kUpdCols) / 256; ++u) {
-        const int idx = tid + 256 * u, jj = idx / kUpdCols, cc = idx % kUpdCols;
+      for (int idx = tid; idx < KC * TL; idx += 256) {
+        const int jj = idx / TL, cc = idx % TL;
         const int j = j0 + jj, l = col0 + cc;
         Cs[jj][cc] = (j < L && l < L) ? C[(long long)j * L + l] : make_double2(0.0, 0.0);
       }
       __syncthreads();
-      const int jn = min(kUpdChunk, L - j0);
+      const int jn = min(KC, L - j0);
       for (int jj = 0; jj < jn; ++jj) {
         double2 w[4], c[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) w[i] = Ws[ty + 16 * i][jj];
+        for (int i = 0; i < 4; ++i) w[i] = Ws[ty + RQ * i][jj];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) c[t] = Cs[jj][tx + 16 * t];
+        for (int t = 0; t < 4; ++t) c[t] = Cs[jj][tx + TQ * t];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -219,11 +240,11 @@ __global__ void __launch_bounds__(256) k_update(V* Y, const V* X, double2 a, con
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const long long row = r0 + ty + 16 * i;
+      const long long row = r0 + ty + RQ * i;
       if (row >= rows) continue;
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        const int l = col0 + tx + 16 * t;
+        const int l = col0 + tx + TQ * t;
         if (l >= L) continue;
         const long long o = row * L + l;
         double2 y = acc[i][t];
@@ -246,13 +267,24 @@ __global__ void __launch_bounds__(256) k_update(V* Y, const V* X, double2 a, con
   }
 }
 
+template <typename V, int TL>
+static void launch_update_tile(V* Y, const V* X, double2 a, const V* W1, const V* W2, double2 w2c,
+                               const double2* C, const V* E, double2 e, long long rows, int L, hipStream_t s) {
+  constexpr int RB = 4096 / TL;
+  const long long slabs = (rows + RB - 1) / RB;
+  const int g = (int)min(slabs, 4096LL);
+  dim3 grid(g, (L + TL - 1) / TL);
+  hipLaunchKernelGGL((k_update<V, TL>), grid, dim3(256), 0, s, Y, X, a, W1, W2, w2c, C, E, e, rows, L);
+}
+
 template <typename V>
 void launch_update(V* Y, const V* X, double2 a, const V* W1, const V* W2, double2 w2c, const double2* C,
                    const V* E, double2 e, long long rows, int L, hipStream_t s) {
-  const long long slabs = (rows + kUpdRows - 1) / kUpdRows;
-  const int g = (int)min(slabs, 4096LL);
-  dim3 grid(g, (L + kUpdCols - 1) / kUpdCols);
-  hipLaunchKernelGGL((k_update<V>), grid, dim3(256), 0, s, Y, X, a, W1, W2, w2c, C, E, e, rows, L);
+  switch (block_tile(L)) {
+    case 16: launch_update_tile<V, 16>(Y, X, a, W1, W2, w2c, C, E, e, rows, L, s); break;
+    case 32: launch_update_tile<V, 32>(Y, X, a, W1, W2, w2c, C, E, e, rows, L, s); break;
+    default: launch_update_tile<V, 64>(Y, X, a, W1, W2, w2c, C, E, e, rows, L, s); break;
+  }
 }
 template void launch_update<double2>(double2*, const double2*, double2, const double2*, const double2*, double2,
                                      const double2*, const double2*, double2, long long, int, hipStream_t);
