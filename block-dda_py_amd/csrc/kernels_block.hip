@@ -108,27 +108,29 @@ __global__ void __launch_bounds__(256) k_gram(const V* __restrict__ A, const V* 
     for (int j = 0; j < 4; ++j) out[(ty + TQ * i) * TL + tx + TQ * j] = acc[i][j];
 }
 
-// out[j][l] = sum of the parts' TL x TL tiles.  A block takes 16
-// consecutive tile entries and 16 part groups: group g sums parts g, g + 16,
-// ..., and the 16 group sums are added in group order (a fixed order:
-// deterministic).  Entries outside the L x L result are skipped.
-constexpr int kRedEntries = 16, kRedGroups = 16;
+// out[j][l] = sum of the parts' TL x TL tiles.  A block of 256 threads takes
+// E consecutive tile entries and G = 256 / E part groups (E chosen by the
+// launcher so that each thread's chain over parts stays short): group g sums
+// parts g, g + G, ..., and the G group sums are added in group order (a
+// fixed order: deterministic).  Entries outside the L x L result are
+// skipped.
 __global__ void __launch_bounds__(256) k_gram_reduce(const double2* __restrict__ partial, int nparts, int ntile,
-                                                     int TL, int L, double2* __restrict__ out) {
-  __shared__ double2 sh[kRedGroups][kRedEntries];
-  const int e = threadIdx.x % kRedEntries, g = threadIdx.x / kRedEntries;
-  const long long id = (long long)blockIdx.x * kRedEntries + e;
+                                                     int TL, int L, int E, double2* __restrict__ out) {
+  __shared__ double2 sh[256];
+  const int G = 256 / E;
+  const int e = threadIdx.x % E, g = threadIdx.x / E;
+  const long long id = (long long)blockIdx.x * E + e;
   const int te = TL * TL, ntile2 = ntile * ntile;
   const int tile = (int)(id / te), w = (int)(id % te);
   const int j = (tile / ntile) * TL + w / TL, l = (tile % ntile) * TL + w % TL;
   const bool live = id < (long long)ntile2 * te && j < L && l < L;
   double2 s = make_double2(0.0, 0.0);
   if (live)
-    for (int p = g; p < nparts; p += kRedGroups) s = zadd(s, partial[((long long)p * ntile2 + tile) * te + w]);
-  sh[g][e] = s;
+    for (int p = g; p < nparts; p += G) s = zadd(s, partial[((long long)p * ntile2 + tile) * te + w]);
+  sh[g * E + e] = s;
   __syncthreads();
   if (g == 0 && live) {
-    for (int q = 1; q < kRedGroups; ++q) s = zadd(s, sh[q][e]);
+    for (int q = 1; q < G; ++q) s = zadd(s, sh[q * E + e]);
     out[j * L + l] = s;
   }
 }
@@ -147,9 +149,14 @@ static void launch_gram_tile(const V* A, const V* B, long long rows, int L, bool
     hipLaunchKernelGGL((k_gram<V, true, TL>), grid, dim3(256), 0, s, A, B, rows, L, partial);
   else
     hipLaunchKernelGGL((k_gram<V, false, TL>), grid, dim3(256), 0, s, A, B, rows, L, partial);
+  // reduction block shape: about 32 parts per thread's chain
+  const int parts = nparts * NG;
+  int G = 1;
+  while (G < 256 && parts / G > 32) G *= 2;
+  const int E = 256 / G;
   const long long entries = (long long)ntile * ntile * TL * TL;
-  hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)((entries + kRedEntries - 1) / kRedEntries)),
-                     dim3(kRedEntries * kRedGroups), 0, s, partial, nparts * NG, ntile, TL, L, out);
+  hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)((entries + E - 1) / E)), dim3(256), 0, s, partial, parts, ntile,
+                     TL, L, E, out);
 }
 
 // The partial buffer holds nparts * 4096 * ceil(L / 64)^2 entries, which
