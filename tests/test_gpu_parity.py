@@ -242,8 +242,10 @@ def _gpu_dd(prob, solver="bicgstab", tol=1e-2, itermax=25, precision="fp64", **k
 
 
 @pytest.mark.parametrize("solver", ["cocg_rq", "bicgstab"])
-@pytest.mark.parametrize("L", [6, 70])
+@pytest.mark.parametrize("L", [6, 24, 70])
 def test_solver_iterates_and_observables_match_oracle(solver, L):
+    """Block solves against the oracle; L = 6, 24 and 70 put the Gram and
+    update kernels on their 16-, 32- and 64-wide tiles (two of them at 70)."""
     say(f"solver {solver} L={L}")
     prob = _problem(L=L)
     ref = _oracle_solve(prob, "cocg" if solver == "cocg_rq" else "bicgstab", 1e-8, 60)
@@ -254,7 +256,7 @@ def test_solver_iterates_and_observables_match_oracle(solver, L):
         np.testing.assert_allclose(dd.err_history, ref["history"], rtol=1e-6)
         assert rel(dd.X, ref["X"]) < 1e-9
     else:
-        # 70 orientations on a ~500-dipole target make the L x L block
+        # 24 or 70 orientations on a ~500-dipole target make the L x L block
         # coefficient systems ill-conditioned, so rounding differences between
         # any two implementations grow along the iteration (the reference has
         # the same property); the converged solutions agree to the tolerance
