@@ -305,6 +305,13 @@ class FNO:
         meta = np.array([self.G, self.C, self.K, self.nl, self.cin, self.cout, self.hid], dtype=np.int64)
         np.savez_compressed(path, __meta__=meta, **{k: v.astype(np.float16) for k, v in self.state().items()})
 
+    def at_grid(self, G):
+        """The same network on a G^3 cube: the spectral weights are indexed by
+        mode (0..5 and G-6..G-1 per axis), so they carry over to any G >= 2
+        modes; only the truncated DFT matrices change."""
+        return FNO(G=G, width=self.C, modes=self.K, layers=self.nl, cin=self.cin, cout=self.cout,
+                   hidden=self.hid, params={k: np.array(v, dtype=np.float32) for k, v in self.state().items()})
+
     @classmethod
     def load(cls, path):
         d = np.load(path)

@@ -78,14 +78,17 @@ def build_inputs(lib, occ_dev, inst_dev, desc_dev, G, n):
 
 
 def random_dipoles(rng, bank, G, L):
-    """One random training particle as a DiscreteDipoles with B on the device."""
+    """One random particle that fits a G^3 cube, as a DiscreteDipoles with B
+    on the device.  r_v is drawn from [0.12, 0.45] (G / 32)^2: the lattice's
+    cells per axis grow as sqrt(r_v) (the spacing grows as sqrt(c)), so the
+    particles fill the cube alike at every G (the training set is G = 32)."""
     from bl_dda.scatterer import DiscreteDipoles, IncidentField, Target
     from shape_model.gaussian_ellipsoid import gaussian_ellipsoid_shape_model as Shape
     DiscreteDipoles.verbose_default = False
     for _ in range(50):
         bc, ab = rng.choice([1.0, 1.25, 1.5]), rng.choice([1.0, 1.25, 1.5])
         beta = rng.uniform(0.0, 0.2)
-        r_v = rng.uniform(0.12, 0.45)
+        r_v = rng.uniform(0.12, 0.45) * (G / 32.0) ** 2
         sh = Shape(r_v, bc, ab, beta)
         pts, _ = sh.surface_from_sample(bank.sample_field(sh))
         _, n, grid = sh.create_cuboid_lattice_that_encloses_GRE_shape(pts)

@@ -10,7 +10,8 @@
 3. compares mixed-precision block-COCG-rq on held-out isotropic particles
    (L = 64) from a zero start, from the network's raw guess and from its
    minimal-residual projection: iterations and time to relative residual
-   1e-5, inference and projection time included.
+   1e-5, inference and projection time included.  `--eval-only --eval-grid
+   64` runs the trained weights on particles that fill a 64^3 cube.
 Prints one JSON summary line.
 """
 import argparse
@@ -129,6 +130,8 @@ def main():
     ap.add_argument("--eval", type=int, default=4)
     ap.add_argument("--out", default=os.path.join(HERE, "bdda_amd", "data", "fno_gre32.npz"))
     ap.add_argument("--eval-only", action="store_true", help="only measure the gain of the weights at --out")
+    ap.add_argument("--eval-grid", type=int, default=0,
+                    help="with --eval-only: run the network on held-out particles of this cube size (0: its own)")
     ap.add_argument("--guess-study", action="store_true", help="iterations saved by guesses of known error")
     a = ap.parse_args()
     from bdda_amd.fno import FNO
@@ -140,7 +143,10 @@ def main():
         return
     if a.eval_only:
         model = FNO.load(a.out)
-        print(json.dumps(dict(weights=a.out, gain=gain(model, n_eval=a.eval, G=a.grid))), flush=True)
+        G = a.eval_grid or model.G
+        if G != model.G:
+            model = model.at_grid(G)
+        print(json.dumps(dict(weights=a.out, grid=G, gain=gain(model, n_eval=a.eval, G=G))), flush=True)
         return
     t0 = time.time()
     data = generate(a.instances, G=a.grid, L=a.orientations, log=log)
