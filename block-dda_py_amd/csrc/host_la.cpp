@@ -1,8 +1,8 @@
 // Small dense complex float64 linear algebra on the host for the L x L
 // recurrence coefficients of the block-Krylov solvers (declarations and
 // conventions in host_la.h).  Plain C++ (built with the host compiler), so
-// each routine can carry an AVX2/FMA clone that the loader selects at
-// run time, with a generic x86-64 fallback.
+// each routine can carry AVX-512 and AVX2/FMA clones that the loader selects
+// at run time, with a generic x86-64 fallback.
 #include "host_la.h"
 
 #include <cmath>
@@ -47,9 +47,9 @@ inline void split_axpy_neg(double* __restrict yr, double* __restrict yi, double 
 
 }  // namespace
 
-// Each routine carries an AVX2/FMA clone, selected by the loader on hosts
-// that have it, and a generic x86-64 one.
-#define BDDA_HOST_CLONES __attribute__((target_clones("arch=haswell", "default")))
+// Each routine carries AVX-512 and AVX2/FMA clones, selected by the loader
+// on hosts that have them, and a generic x86-64 one.
+#define BDDA_HOST_CLONES __attribute__((target_clones("avx512f", "arch=haswell", "default")))
 
 // In-place LU with partial pivoting (row interchanges recorded in piv);
 // returns false if exactly singular.
@@ -128,34 +128,66 @@ BDDA_HOST_CLONES void lu_solve(const std::vector<zc>& LU, const std::vector<int>
 }
 
 // Hermitian positive definite G = R^H R with R upper triangular; false if not PD.
+// Right-looking, one row at a time: row j of R is row j of the updated
+// matrix over r_jj, then every later row i takes its update
+// -conj(R[j][i]) R[j][i..n) (contiguous, vectorises).  Each entry still
+// accumulates its products in the order j = 0, 1, ...
 BDDA_HOST_CLONES bool cholesky_upper(const std::vector<zc>& G, std::vector<zc>& R, int n) {
-  R.assign((size_t)n * n, zc(0.0));
+  Split A;
+  A.load(G);
+  double* ar = A.re.data();
+  double* ai = A.im.data();
   for (int j = 0; j < n; ++j) {
-    double d = G[j * n + j].real();
-    for (int k = 0; k < j; ++k) d -= std::norm(R[k * n + j]);
+    const double d = ar[j * n + j];
     if (!(d > 0.0)) return false;
-    const double rjj = std::sqrt(d);
-    R[j * n + j] = rjj;
+    const double rjj = std::sqrt(d), inv = 1.0 / rjj;
+    ar[j * n + j] = rjj;
+    ai[j * n + j] = 0.0;
+    for (int l = j + 1; l < n; ++l) {
+      ar[j * n + l] *= inv;
+      ai[j * n + l] *= inv;
+    }
     for (int i = j + 1; i < n; ++i) {
-      zc s = G[j * n + i];
-      for (int k = 0; k < j; ++k) s -= mul(std::conj(R[k * n + j]), R[k * n + i]);
-      R[j * n + i] = s / rjj;
+      // row i from column i on: A[i][l] -= conj(R[j][i]) R[j][l]
+      const double cr = ar[j * n + i], ci = -ai[j * n + i];
+      split_axpy_neg(ar + i * n + i, ai + i * n + i, cr, ci, ar + j * n + i, ai + j * n + i, n - i);
     }
   }
+  R.assign((size_t)n * n, zc(0.0));
+  for (int j = 0; j < n; ++j)
+    for (int l = j; l < n; ++l) R[j * n + l] = zc(ar[j * n + l], ai[j * n + l]);
   return true;
 }
 
-// Inverse of an upper triangular matrix.
+// Inverse of an upper triangular matrix, a row at a time from the bottom:
+// row i of R^-1 is -(1 / R_ii) sum_{k > i} R_ik row k, plus 1 / R_ii on the
+// diagonal (the row updates are contiguous and vectorise).
 BDDA_HOST_CLONES void upper_inverse(const std::vector<zc>& R, std::vector<zc>& Ri, int n) {
-  Ri.assign((size_t)n * n, zc(0.0));
-  for (int j = 0; j < n; ++j) {
-    Ri[j * n + j] = 1.0 / R[j * n + j];
-    for (int i = j - 1; i >= 0; --i) {
-      zc s = 0.0;
-      for (int k = i + 1; k <= j; ++k) s += mul(R[i * n + k], Ri[k * n + j]);
-      Ri[i * n + j] = -s / R[i * n + i];
+  Split F, X;
+  F.load(R);
+  X.re.assign((size_t)n * n, 0.0);
+  X.im.assign((size_t)n * n, 0.0);
+  const double* fr = F.re.data();
+  const double* fi = F.im.data();
+  double* xr = X.re.data();
+  double* xi = X.im.data();
+  for (int i = n - 1; i >= 0; --i) {
+    for (int k = i + 1; k < n; ++k) {
+      // X[i][k..n) += R[i][k] X[k][k..n)
+      split_axpy_neg(xr + i * n + k, xi + i * n + k, -fr[i * n + k], -fi[i * n + k], xr + k * n + k,
+                     xi + k * n + k, n - k);
     }
+    const zc inv = 1.0 / zc(fr[i * n + i], fi[i * n + i]);
+    for (int l = i + 1; l < n; ++l) {
+      const zc v = mul(zc(xr[i * n + l], xi[i * n + l]), -inv);
+      xr[i * n + l] = v.real();
+      xi[i * n + l] = v.imag();
+    }
+    xr[i * n + i] = inv.real();
+    xi[i * n + i] = inv.imag();
   }
+  Ri.resize((size_t)n * n);
+  X.store(Ri);
 }
 
 // C = op(A) B  with op = transpose (no conjugate) when ta.
