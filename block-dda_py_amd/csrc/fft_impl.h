@@ -928,6 +928,67 @@ __device__ __forceinline__ void xconv_load(const LinePass& lp, int pt, int lo, i
   }
 }
 
+// Waves per SIMD the x pass is compiled for.  Its blocks are 6 waves, placed
+// 2-2-1-1 over a CU's four SIMDs, so at 5 waves per SIMD (81-96 VGPRs) a
+// third block rarely finds room; the complex128 m = 128 kernel fits 80
+// VGPRs (6 waves per SIMD) without spilling, and then three blocks share a
+// CU.  Other instances keep the compiler's default.
+template <typename V, int R1, int R2, int LB>
+constexpr int xconv_waves_per_simd() {
+  return (sizeof(V) == sizeof(double2) && R1 * R2 == 128 && LB == 8) ? 6 : 2;
+}
+
+// Forward step 1 of the x pass on a half-zero input x[j1 R2 + w], j1 < R1/2,
+// as two half-length DFTs: X[2k] = DFT_{R1/2}(x)[k] and X[2k + 1] =
+// DFT_{R1/2}(x_j W_R1^j)[k].  The inputs wait in this thread's odd-k1 rows
+// while the even half is formed, so only R1/2 values are live at a time
+// (this step sets the kernel's register peak); each output is twiddled by
+// W_M^(w k1) and written to row k1 R2 + w.
+template <typename V, int R1, int R2, int S>
+__device__ __forceinline__ void xconv_step1_even(V* a, V* __restrict__ buf, const V* __restrict__ tws, int w, int q) {
+  constexpr int H = R1 / 2;
+  constexpr int GRP = 4;  // twiddle loads in flight per group
+  V* __restrict__ col = buf + w * S + q;
+#pragma unroll
+  for (int j = 0; j < H; ++j) col[(2 * j + 1) * R2 * S] = a[j];
+  Dft<H, false, V>::run(a);
+  col[0] = a[0];
+#pragma unroll
+  for (int k0 = 1; k0 < H; k0 += GRP) {
+    V t[GRP];
+#pragma unroll
+    for (int k = k0; k < k0 + GRP && k < H; ++k) t[k - k0] = tws[2 * k * w];
+#pragma unroll
+    for (int k = k0; k < k0 + GRP && k < H; ++k) col[2 * k * R2 * S] = cmul(a[k], t[k - k0]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+// The odd half: x_j W_R1^j (compile-time twiddles), DFT, twiddle, store.
+template <typename V, int R1, int R2, int S>
+__device__ __forceinline__ void xconv_step1_odd(V* a, V* __restrict__ buf, const V* __restrict__ tws, int w, int q) {
+  using T = typename RealOf<V>::T;
+  constexpr int H = R1 / 2;
+  constexpr int GRP = 4;
+  constexpr TwiddleR<R1> w1{};
+  V* __restrict__ col = buf + w * S + q;
+#pragma unroll
+  for (int j = 0; j < H; ++j) {
+    const V x = col[(2 * j + 1) * R2 * S];
+    const T c = (T)w1.c[j], sn = (T)w1.s[j];
+    a[j] = j == 0 ? x : cx<V>(x.x * c + x.y * sn, x.y * c - x.x * sn);
+  }
+  Dft<H, false, V>::run(a);
+#pragma unroll
+  for (int k0 = 0; k0 < H; k0 += GRP) {
+    V t[GRP];
+#pragma unroll
+    for (int k = k0; k < k0 + GRP && k < H; ++k) t[k - k0] = tws[(2 * k + 1) * w];
+#pragma unroll
+    for (int k = k0; k < k0 + GRP && k < H; ++k) col[(2 * k + 1) * R2 * S] = cmul(a[k], t[k - k0]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
 // One x line tile after its inputs are in a[0, R1/2): forward transform, 3x3
 // Green multiply, inverse transform, store.  Needs the twiddles staged in
 // tws; buf is free again only after a barrier.
@@ -943,18 +1004,30 @@ __device__ __forceinline__ void xconv_point(const LinePass& lp, V* buf, const V*
   constexpr int J1 = R1 / 2;
   const int q = tid % Q, w = tid / Q;
   const bool s1 = w < R2, s2 = w < R1;
-#pragma unroll
-  for (int j1 = J1; j1 < R1; ++j1) a[j1] = cx<V>(0, 0);
-  if (s1) {
-    reg2_step1<R1, R2, false>(a, tws, w);
-#pragma unroll
-    for (int k1 = 0; k1 < R1; ++k1) buf[(k1 * R2 + w) * S + q] = a[k1];
-  }
   // Green entries of this thread's multiply row, in flight during step 2
   const int r = tid % M, grp = tid / M;
   V gr[6];
-  green_row<V>(lp, pt, r / R2 + R1 * (r % R2), gr);
-  __syncthreads();
+  if constexpr (xconv_waves_per_simd<V, R1, R2, LB>() > 2) {
+    // register-lean form: split step 1; the Green loads are issued only
+    // after it (not hoisted into it), in flight over the barrier
+    if (s1) {
+      xconv_step1_even<V, R1, R2, S>(a, buf, tws, w, q);
+      xconv_step1_odd<V, R1, R2, S>(a, buf, tws, w, q);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    green_row<V>(lp, pt, r / R2 + R1 * (r % R2), gr);
+    __syncthreads();
+  } else {
+#pragma unroll
+    for (int j1 = J1; j1 < R1; ++j1) a[j1] = cx<V>(0, 0);
+    if (s1) {
+      reg2_step1<R1, R2, false>(a, tws, w);
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) buf[(k1 * R2 + w) * S + q] = a[k1];
+    }
+    green_row<V>(lp, pt, r / R2 + R1 * (r % R2), gr);
+    __syncthreads();
+  }
   if (s2) {
 #pragma unroll
     for (int j2 = 0; j2 < R2; ++j2) a[j2] = buf[(w * R2 + j2) * S + q];
@@ -1024,7 +1097,8 @@ __device__ __forceinline__ void xconv_item(const LinePass& lp, int item, int* pt
 }
 
 template <typename V, int R1, int R2, int LB>
-__global__ void __launch_bounds__(512) k_xconv(LinePass lp) {
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(xconv_waves_per_simd<V, R1, R2, LB>())))
+k_xconv(LinePass lp) {
   constexpr int M = R1 * R2;
   constexpr int RMAX = R1 > R2 ? R1 : R2;
   constexpr int Q = 3 * LB;
