@@ -116,6 +116,11 @@ class Plan:
         posa = None if pos is None else np.ascontiguousarray(pos, dtype=np.float64)
         check(self.lib.bdda_set_target(self.h, ptr(address), N, ptr(posa), ptr(diag), ptr(jpre)), "set_target")
         self.N = N
+        # occupied extent (ix, iy, iz bounds of the occupied cells): the
+        # operator's passes skip the empty lines and rows beyond it
+        ny, nz = self.n[1], self.n[2]
+        self.occupied_extent = (int(address.max() // (ny * nz)) + 1, int((address // nz % ny).max()) + 1,
+                                int((address % nz).max()) + 1)
         self.target_key = key
         self._jpre_dev = torch.as_tensor(jpre, device="cuda")
 

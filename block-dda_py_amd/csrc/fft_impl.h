@@ -58,6 +58,13 @@ template <typename V> __device__ __forceinline__ V load_or_zero(bool c, const V*
   if (c) v = *p;
   return v;
 }
+// First cell (iz = 0) of z line pt of a z pass.  The pass's lines are the
+// (ix, iy) of a rectangle at the cuboid's origin, ptb lines along y (the
+// occupied extent, or Ny); cells are numbered (ix Ny + iy) Nz + iz.
+__device__ __forceinline__ long long zline_cell(const LinePass& lp, int pt) {
+  return ((long long)(pt / lp.ptb) * lp.nyz_y + pt % lp.ptb) * lp.nz;
+}
+
 template <typename V> __device__ __forceinline__ V cadd(V a, V b) { return cx<V>(a.x + b.x, a.y + b.y); }
 template <typename V> __device__ __forceinline__ V csub(V a, V b) { return cx<V>(a.x - b.x, a.y - b.y); }
 template <typename V> __device__ __forceinline__ V cmul(V a, V b) {
@@ -447,7 +454,7 @@ __global__ void __launch_bounds__(512) k_line_pass(LinePass lp) {
       const int e = tm.rt + i * tm.TPC;
       mrow[i] = -1;
       if (active && e < lp.nin && l < lp.L) {
-        const long long cell = (long long)pt * lp.nz + e;
+        const long long cell = zline_cell(lp, pt) + e;
         mrow[i] = is_scatter<MODE>() ? (long long)lp.inv[cell] : cell;
       }
     }
@@ -498,7 +505,7 @@ __global__ void __launch_bounds__(512) k_line_pass(LinePass lp) {
       const int e = tm.rt + i * tm.TPC;
       row[i] = -1;
       if (e < lp.nout) {
-        const long long cell = (long long)pt * lp.nz + e;
+        const long long cell = zline_cell(lp, pt) + e;
         const long long m = is_gather<MODE>() ? (long long)lp.inv[cell] : cell;
         if (m >= 0) row[i] = 3 * m + c;
       }
@@ -669,7 +676,7 @@ __global__ void __launch_bounds__((reg2_max_threads<V, LB, MODE>())) k_reg2(Line
         const int e = j1 * R2 + j2;
         mrow[j1] = -1;
         if (s1 && active && e < lp.nin && l < lp.L) {
-          const long long cell = (long long)pt * lp.nz + e;
+          const long long cell = zline_cell(lp, pt) + e;
           mrow[j1] = is_scatter<MODE>() ? (long long)lp.inv[cell] : cell;
         }
       }
@@ -703,7 +710,7 @@ __global__ void __launch_bounds__((reg2_max_threads<V, LB, MODE>())) k_reg2(Line
       for (int k2 = 0; k2 < R2; ++k2) {
         const int e = k1 + R1 * k2;
         const bool ok = s2 && active && e < lp.nout;
-        outm[k2] = ok ? lp.inv[(long long)pt * lp.nz + e] : -1;
+        outm[k2] = ok ? lp.inv[zline_cell(lp, pt) + e] : -1;
       }
     }
     const V twv = load_or_zero(tid < M, reinterpret_cast<const V*>(lp.tw) + tid);
@@ -777,7 +784,7 @@ __global__ void __launch_bounds__((reg2_max_threads<V, LB, MODE>())) k_reg2(Line
 #pragma unroll
           for (int k2 = 0; k2 < R2; ++k2) {
             const int e = k1 + R1 * k2;
-            const int m = is_gather<MODE>() ? outm[k2] : (e < lp.nout ? pt * lp.nz + e : -1);
+            const int m = is_gather<MODE>() ? outm[k2] : (e < lp.nout ? (int)zline_cell(lp, pt) + e : -1);
             row[k2] = m >= 0 ? 3 * m + c : -1;
           }
           if (is_gather<MODE>()) {

@@ -94,6 +94,9 @@ struct bdda_plan {
   bool target_ready = false;
   bool has_pos = false;
   long long N = 0;
+  // occupied extent of the target: every occupied cell has ix < occ[0],
+  // iy < occ[1], iz < occ[2] (the cuboid's dims until a target is set)
+  int occ[3] = {0, 0, 0};
   // twiddles
   bdda::DevBuf tw64[3], tw32[3];
   // Green half spectrum
@@ -217,6 +220,15 @@ static int apply_operator(bdda_plan* p, const BV* S, BV* AS, int L, bool occupie
   const int nlo = (L + Lb - 1) / Lb;
   const int nx = p->ax[0].n, ny = p->ax[1].n, nz = p->ax[2].n;
   const int mx = p->ax[0].m, my = p->ax[1].m, mz = p->ax[2].m;
+  // Occupied blocks: cells beyond the target's occupied extent are empty, so
+  // the lines and rows there hold only zeros and every pass skips them (a
+  // cuboid padded beyond the lattice costs nothing); x runs to ox, y to oy
+  // and z to oz.  T1 lines and T2 rows outside are neither written nor read.
+  const int ox = occupied ? p->occ[0] : nx, oy = occupied ? p->occ[1] : ny, oz = occupied ? p->occ[2] : nz;
+  if (ox < 1 || oy < 1 || oz < 1 || ox > nx || oy > ny || oz > nz) {
+    set_error("apply_operator: occupied extent outside the cuboid");
+    return BDDA_E_STATE;
+  }
   int rc = ensure_fft_ws<V>(p, L);
   if (rc) return rc;
   g_err.clear();
@@ -253,10 +265,11 @@ static int apply_operator(bdda_plan* p, const BV* S, BV* AS, int L, bool occupie
     LinePass lp = base;
     axis(lp, 2);
     lp.mode = occupied ? (wide ? PASS_SCATTER_Z64 : PASS_SCATTER_Z) : PASS_CUBOID_Z;
-    lp.nin = nz;
+    lp.nin = oz;
     lp.nout = mz;
-    lp.npts = nx * ny;
-    lp.ptb = lp.npts;
+    lp.npts = ox * oy;
+    lp.ptb = oy;
+    lp.da = (long long)ny * mz * Q;
     lp.db = (long long)mz * Q;
     lp.de = Q;
     lp.lo_dst = t1_lo;
@@ -270,9 +283,9 @@ static int apply_operator(bdda_plan* p, const BV* S, BV* AS, int L, bool occupie
     LinePass lp = base;
     axis(lp, 1);
     lp.mode = PASS_PLAIN;
-    lp.nin = ny;
+    lp.nin = oy;
     lp.nout = my;
-    lp.npts = nx * mz;
+    lp.npts = ox * mz;
     lp.ptb = mz;
     lp.sa = (long long)ny * mz * Q;
     lp.sb = Q;
@@ -293,8 +306,8 @@ static int apply_operator(bdda_plan* p, const BV* S, BV* AS, int L, bool occupie
     LinePass lp = base;
     axis(lp, 0);
     lp.mode = PASS_CONV_X;
-    lp.nin = nx;
-    lp.nout = nx;
+    lp.nin = ox;
+    lp.nout = ox;
     lp.npts = my * mz;
     lp.ptb = lp.npts;
     lp.sb = Q;
@@ -316,8 +329,8 @@ static int apply_operator(bdda_plan* p, const BV* S, BV* AS, int L, bool occupie
     lp.mode = PASS_PLAIN;
     lp.inverse = 1;
     lp.nin = my;
-    lp.nout = ny;
-    lp.npts = nx * mz;
+    lp.nout = oy;
+    lp.npts = ox * mz;
     lp.ptb = mz;
     lp.sa = (long long)my * mz * Q;
     lp.sb = Q;
@@ -340,9 +353,10 @@ static int apply_operator(bdda_plan* p, const BV* S, BV* AS, int L, bool occupie
     lp.mode = occupied ? (wide ? PASS_GATHER_Z64 : PASS_GATHER_Z) : PASS_CROP_Z;
     lp.inverse = 1;
     lp.nin = mz;
-    lp.nout = nz;
-    lp.npts = nx * ny;
-    lp.ptb = lp.npts;
+    lp.nout = oz;
+    lp.npts = ox * oy;
+    lp.ptb = oy;
+    lp.sa = (long long)ny * mz * Q;
     lp.sb = (long long)mz * Q;
     lp.se = Q;
     lp.lo_src = t1_lo;
@@ -994,6 +1008,17 @@ int bdda_set_target(bdda_plan* p, const int64_t* address, int64_t N, const doubl
       set_error("bdda_set_target: addresses must be increasing and inside the cuboid");
       return BDDA_E_ARG;
     }
+  }
+  {
+    const long long ny = p->ax[1].n, nz = p->ax[2].n;
+    int ext[3] = {0, 0, 0};
+    for (int64_t i = 0; i < N; ++i) {
+      const long long a = address[i];
+      ext[0] = std::max(ext[0], (int)(a / (ny * nz)) + 1);
+      ext[1] = std::max(ext[1], (int)((a / nz) % ny) + 1);
+      ext[2] = std::max(ext[2], (int)(a % nz) + 1);
+    }
+    for (int d = 0; d < 3; ++d) p->occ[d] = ext[d];
   }
   int rc = p->inv.ensure(p->ncell * sizeof(int));
   if (!rc) rc = p->pos.ensure(N * 3 * sizeof(double));

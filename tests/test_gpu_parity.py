@@ -161,6 +161,40 @@ def test_block_mvp_lengths_match_oracle(n, L):
     assert rel(pm.fft_mvp(n, 3, At, P, precision="mixed"), ref) < 2e-6
 
 
+@pytest.mark.parametrize("n,L,box", [((20, 18, 16), 8, (7, 5, 16)), ((16, 24, 20), 8, (16, 9, 3)),
+                                     ((12, 12, 12), 5, (1, 1, 1)), ((40, 6, 5), 16, (13, 6, 5)),
+                                     ((64, 6, 5), 8, (44, 3, 5)), ((9, 10, 96), 8, (4, 10, 50))])
+def test_block_mvp_on_a_confined_target_matches_oracle(n, L, box):
+    """Every occupied cell inside the box at the cuboid's origin: the passes
+    then skip the lines and rows beyond it (x rows, y rows, z lines), and the
+    product must still equal the oracle's _block_mvp over the whole cuboid.
+    The cases take the dedicated x pass (lengths 80 and 128, the latter in
+    its register-lean form), the register z pass on length 192, and a
+    runtime tile."""
+    say(f"confined block mvp n={n} L={L} box={box}")
+    from bl_krylov import bl_krylov as pk
+    from mvp_fft import mvp_fft as pm
+    from bdda_amd.plan import Plan
+    lf, k = 0.03, 9.5
+    rng = np.random.default_rng(17)
+    bx, by, bz = box
+    cells = np.array([(ix * n[1] + iy) * n[2] + iz for ix in range(bx) for iy in range(by) for iz in range(bz)])
+    pick = rng.choice(cells.size, size=max(1, cells.size // 3), replace=False)
+    corner = ((bx - 1) * n[1] + by - 1) * n[2] + bz - 1
+    address = np.unique(np.append(cells[pick], corner)).astype(np.int64)
+    N = address.size
+    diag = rng.standard_normal(3 * N) + 1j * rng.standard_normal(3 * N) + 3.0
+    jpre = 1.0 / diag
+    S = rng.standard_normal((3 * N, L)) + 1j * rng.standard_normal((3 * N, L))
+    idx = pk._build_scatter_index(address, 3)
+    a = pk._block_mvp(n, 3, pm.fft_init(n, 3, lf, k), idx, diag, S, jpre)
+    b = okr.block_mvp(n, 3, omvp.fft_init(n, 3, lf, k), idx, diag, S, jpre)
+    assert rel(a, b) < 1e-12
+    pl = Plan(n)
+    pl.set_target(address, diag, jpre)
+    assert pl.occupied_extent == box
+
+
 def test_column_tile_prefers_the_compiled_8_column_tile():
     """Among equally padded tiles the operator takes 8 columns (the register
     kernels' compile-time tile); a tile that pads L less wins otherwise."""
