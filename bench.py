@@ -66,26 +66,29 @@ def measured_traffic():
     return int(m.group(1)) if m else None
 
 
-def pass_bytes(dims, m, N, L, Lb, s, extent=None):
+def pass_bytes(dims, m, N, L, Lb, s, extent=None, lines=None):
     """Algorithmic HBM bytes of each FFT pass for one product (each pass reads
     and writes its pruned array once; the x pass also reads the 6-component
-    half spectrum once).  With `extent` (the occupied extent ox, oy, oz) the
-    count covers the lines and rows the passes process: x rows below ox,
-    y rows below oy; without it, the whole cuboid."""
+    half spectrum once).  With `extent` (the occupied extent ox, oy, oz) and
+    `lines` (the z lines holding an occupied cell) the count covers what the
+    passes process: x rows below ox, y rows below oy, and only the occupied
+    z lines in the z passes and the y inverse pass's stores; without them,
+    the whole cuboid."""
     nx, ny, nz = extent if extent is not None else dims
     mx, my, mz = m
     Lp = -(-L // Lb) * Lb
     c = 3 * Lp
     t1 = nx * ny * mz * c
+    t1_occ = (lines if lines is not None else nx * ny) * mz * c
     t2 = nx * my * mz * c
     half = (mx // 2 + 1) * (my // 2 + 1) * (mz // 2 + 1) * 6
     occ = 3 * N * L
     return {
-        "z_fwd": s * (occ + t1),
+        "z_fwd": s * (occ + t1_occ),
         "y_fwd": s * (t1 + t2),
         "x_conv": s * (2 * t2 + half),
-        "y_inv": s * (t2 + t1),
-        "z_inv": s * (t1 + 2 * occ) + 2 * 16 * 3 * N,
+        "y_inv": s * (t2 + t1_occ),
+        "z_inv": s * (t1_occ + 2 * occ) + 2 * 16 * 3 * N,
     }
 
 
@@ -264,8 +267,8 @@ def main():
         f"(checking it: {dt_check * 1e3:.2f} ms)")
 
     Lb = plan.column_tile(L)
-    ext = dd._plan.occupied_extent
-    pb = pass_bytes(dims, m, N, L, Lb, 16, ext)
+    ext, nlines = dd._plan.occupied_extent, dd._plan.occupied_lines
+    pb = pass_bytes(dims, m, N, L, Lb, 16, ext, nlines)
     passes = {}
     for nm in PASSES:
         ms, cnt = prof.get(("fp64", nm), (0.0, 0))
@@ -305,8 +308,9 @@ def main():
         "roofline": roofline,
         "mvp": {"ms": round(mvp_ms, 4), "alg_bytes": mvp_alg, "alg_GBs": round(mvp_alg / mvp_ms / 1e6, 1),
                 "alg_bytes_note": "16 (26 n 3L + 6 M) over the whole cuboid; the passes skip the lines and rows "
-                                  "beyond the target's occupied extent, so they move bytes_processed",
-                "occupied_extent": list(ext), "bytes_processed": mvp_done,
+                                  "beyond the target's occupied extent and the z lines with no occupied cell, "
+                                  "so they move bytes_processed",
+                "occupied_extent": list(ext), "occupied_z_lines": nlines, "bytes_processed": mvp_done,
                 "GBs_processed": round(mvp_done / mvp_ms / 1e6, 1),
                 "passes": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                            for k, v in passes.items()}},
@@ -319,7 +323,7 @@ def main():
         dtm, profm = timed_solves(dd, "mixed", a.steps, a.warmup, torch)
         S_fwm = np.stack(dd.compute_PCAS_observable_S_fw())
         S_bkm = dd.compute_OCBS_observable_S_bk()
-        pbm = pass_bytes(dims, m, N, L, Lb, 8, ext)
+        pbm = pass_bytes(dims, m, N, L, Lb, 8, ext, nlines)
         mp = {}
         for nm in PASSES:
             ms, cnt = profm.get(("fp32", nm), (0.0, 0))

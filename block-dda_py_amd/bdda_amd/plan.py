@@ -121,6 +121,8 @@ class Plan:
         ny, nz = self.n[1], self.n[2]
         self.occupied_extent = (int(address.max() // (ny * nz)) + 1, int((address // nz % ny).max()) + 1,
                                 int((address % nz).max()) + 1)
+        # z lines (ix, iy) holding an occupied cell: the z passes run only these
+        self.occupied_lines = int(np.unique(address // nz).size)
         self.target_key = key
         self._jpre_dev = torch.as_tensor(jpre, device="cuda")
 

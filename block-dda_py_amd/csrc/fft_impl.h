@@ -49,6 +49,10 @@ template <typename V> struct BoundaryOf<V, PASS_SCATTER_Z64> { using T = double2
 template <typename V> struct BoundaryOf<V, PASS_GATHER_Z64> { using T = double2; };
 template <int MODE> constexpr bool is_scatter() { return MODE == PASS_SCATTER_Z || MODE == PASS_SCATTER_Z64; }
 template <int MODE> constexpr bool is_gather() { return MODE == PASS_GATHER_Z || MODE == PASS_GATHER_Z64; }
+template <int MODE>
+constexpr bool is_z_pass() {
+  return is_scatter<MODE>() || is_gather<MODE>() || MODE == PASS_CUBOID_Z || MODE == PASS_CROP_Z;
+}
 
 // Guarded load returning a value: `c ? p[i] : zero` with two lvalues would
 // make the compiler load through a pointer select (a flat load that may hit a
@@ -58,11 +62,23 @@ template <typename V> __device__ __forceinline__ V load_or_zero(bool c, const V*
   if (c) v = *p;
   return v;
 }
-// First cell (iz = 0) of z line pt of a z pass.  The pass's lines are the
-// (ix, iy) of a rectangle at the cuboid's origin, ptb lines along y (the
-// occupied extent, or Ny); cells are numbered (ix Ny + iy) Nz + iz.
-__device__ __forceinline__ long long zline_cell(const LinePass& lp, int pt) {
-  return ((long long)(pt / lp.ptb) * lp.nyz_y + pt % lp.ptb) * lp.nz;
+// First cell (iz = 0) of z line zl = ix Ny + iy (cells are numbered
+// zl Nz + iz).
+__device__ __forceinline__ long long zline_cell(const LinePass& lp, int zl) { return (long long)zl * lp.nz; }
+
+// Z line run by point pt of a scatter / gather z pass (an occupied-line list
+// when given; the cuboid passes run every line).
+template <int MODE>
+__device__ __forceinline__ int zline_of(const LinePass& lp, int pt, bool active) {
+  if constexpr (is_scatter<MODE>() || is_gather<MODE>())
+    return (lp.zlines != nullptr && active) ? lp.zlines[pt] : pt;
+  return pt;
+}
+// Whether the y inverse pass stores element e (y row) of point (ix, kz): T1
+// keeps zeros on the z lines outside the occupied-line mask (all are stored
+// without one).
+__device__ __forceinline__ bool t1_line_held(const LinePass& lp, int ix, int e) {
+  return lp.lmask == nullptr || lp.lmask[ix * lp.nyz_y + e] != 0;
 }
 
 template <typename V> __device__ __forceinline__ V cadd(V a, V b) { return cx<V>(a.x + b.x, a.y + b.y); }
@@ -441,6 +457,7 @@ __global__ void __launch_bounds__(512) k_line_pass(LinePass lp) {
   const int np_here = min(P, lp.npts - pt0);
   const bool active = tm.rt < tm.TPC && p < np_here;
   const int pt = pt0 + p;
+  const int zl = zline_of<MODE>(lp, pt, active);  // the point's line (z passes)
   const V* __restrict__ src = reinterpret_cast<const V*>(lp.src) + (long long)lo * lp.lo_src;
 
   // ---- load: every global load of this thread (inputs, then twiddles) is
@@ -454,7 +471,7 @@ __global__ void __launch_bounds__(512) k_line_pass(LinePass lp) {
       const int e = tm.rt + i * tm.TPC;
       mrow[i] = -1;
       if (active && e < lp.nin && l < lp.L) {
-        const long long cell = zline_cell(lp, pt) + e;
+        const long long cell = zline_cell(lp, zl) + e;
         mrow[i] = is_scatter<MODE>() ? (long long)lp.inv[cell] : cell;
       }
     }
@@ -462,7 +479,7 @@ __global__ void __launch_bounds__(512) k_line_pass(LinePass lp) {
 #pragma unroll
     for (int i = 0; i < VPT; ++i) v[i] = cvt<V>(load_or_zero(mrow[i] >= 0, S + (3 * mrow[i] + c) * lp.L + l));
   } else {
-    const int a = pt / lp.ptb, b = pt % lp.ptb;
+    const int a = zl / lp.ptb, b = zl % lp.ptb;
     const V* __restrict__ base = src + a * lp.sa + b * lp.sb + q;
 #pragma unroll
     for (int i = 0; i < VPT; ++i) {
@@ -505,7 +522,7 @@ __global__ void __launch_bounds__(512) k_line_pass(LinePass lp) {
       const int e = tm.rt + i * tm.TPC;
       row[i] = -1;
       if (e < lp.nout) {
-        const long long cell = zline_cell(lp, pt) + e;
+        const long long cell = zline_cell(lp, zl) + e;
         const long long m = is_gather<MODE>() ? (long long)lp.inv[cell] : cell;
         if (m >= 0) row[i] = 3 * m + c;
       }
@@ -546,12 +563,13 @@ __global__ void __launch_bounds__(512) k_line_pass(LinePass lp) {
   }
   if (!active) return;
   V* __restrict__ dst = reinterpret_cast<V*>(lp.dst) + (long long)lo * lp.lo_dst;
-  const int a = pt / lp.ptb, b = pt % lp.ptb;
+  const int a = zl / lp.ptb, b = zl % lp.ptb;
   V* __restrict__ obase = dst + a * lp.da + b * lp.db + q;
+  const bool ymask = MODE == PASS_PLAIN && lp.inverse;  // y inverse writes T1
 #pragma unroll
   for (int i = 0; i < VPT; ++i) {
     const int e = tm.rt + i * tm.TPC;
-    if (e < lp.nout) obase[e * lp.de] = buf[e * QQ + tm.qq];
+    if (e < lp.nout && (!ymask || t1_line_held(lp, a, e))) obase[e * lp.de] = buf[e * QQ + tm.qq];
   }
 }
 
@@ -606,9 +624,13 @@ __device__ __forceinline__ V green_entry(const LinePass& lp, int pt0, int i, int
 // that; runtime tiles keep 512 and the register budget that comes with it).
 // The z passes' lines are contiguous already (a z line is one run of
 // memory), so they keep 512-thread blocks, which measured faster there.
-template <int MODE>
-constexpr bool is_z_pass() {
-  return is_scatter<MODE>() || is_gather<MODE>() || MODE == PASS_CUBOID_Z || MODE == PASS_CROP_Z;
+// Waves per SIMD the register kernels are compiled for: the complex128 z
+// passes (512-thread blocks, two waves per SIMD each) are held to 128 VGPRs
+// so that two blocks share a CU, on the lengths that fit that without
+// spilling (second radix up to 12).
+template <typename V, int R2, int MODE>
+constexpr int reg2_waves_per_simd() {
+  return (sizeof(V) == sizeof(double2) && is_z_pass<MODE>() && R2 <= 12) ? 4 : 2;
 }
 template <typename V, int LB, int MODE>
 constexpr int reg2_max_threads() {
@@ -627,7 +649,8 @@ constexpr int reg2_points(int M, int RMAX, int Q) {
 // workgroup from reg2_points), so that the tile width and every LDS offset
 // are compile-time constants; LB = 0: tile taken from the launch arguments.
 template <typename V, int R1, int R2, int MODE, int LB>
-__global__ void __launch_bounds__((reg2_max_threads<V, LB, MODE>())) k_reg2(LinePass lp) {
+__global__ void __launch_bounds__((reg2_max_threads<V, LB, MODE>()))
+__attribute__((amdgpu_waves_per_eu(reg2_waves_per_simd<V, R2, MODE>()))) k_reg2(LinePass lp) {
   constexpr int M = R1 * R2;
   constexpr int RMAX = R1 > R2 ? R1 : R2;
   constexpr bool INV_PASS = (is_gather<MODE>() || MODE == PASS_CROP_Z || MODE == PASS_PLAIN_INV);
@@ -659,6 +682,7 @@ __global__ void __launch_bounds__((reg2_max_threads<V, LB, MODE>())) k_reg2(Line
   const int np_here = min(P, lp.npts - pt0);
   const bool active = p < np_here;
   const int pt = pt0 + p;
+  const int zl = zline_of<MODE>(lp, pt, active);  // the point's line (z passes)
   const bool s1 = w < R2, s2 = w < R1;
   const int j2 = w, k1 = w;
   {
@@ -676,7 +700,7 @@ __global__ void __launch_bounds__((reg2_max_threads<V, LB, MODE>())) k_reg2(Line
         const int e = j1 * R2 + j2;
         mrow[j1] = -1;
         if (s1 && active && e < lp.nin && l < lp.L) {
-          const long long cell = zline_cell(lp, pt) + e;
+          const long long cell = zline_cell(lp, zl) + e;
           mrow[j1] = is_scatter<MODE>() ? (long long)lp.inv[cell] : cell;
         }
       }
@@ -685,7 +709,7 @@ __global__ void __launch_bounds__((reg2_max_threads<V, LB, MODE>())) k_reg2(Line
       for (int j1 = 0; j1 < R1; ++j1)
         a[j1] = cvt<V>(load_or_zero(mrow[j1] >= 0, S + (3 * mrow[j1] + c) * lp.L + l));
     } else {
-      const int ga = pt / lp.ptb, gb = pt % lp.ptb;
+      const int ga = zl / lp.ptb, gb = zl % lp.ptb;
       const int se = (int)lp.se;  // a line spans fewer than 2^31 elements (checked at launch)
       const V* __restrict__ base = src + ga * lp.sa + gb * lp.sb + q + j2 * se;
 #pragma unroll
@@ -710,7 +734,7 @@ __global__ void __launch_bounds__((reg2_max_threads<V, LB, MODE>())) k_reg2(Line
       for (int k2 = 0; k2 < R2; ++k2) {
         const int e = k1 + R1 * k2;
         const bool ok = s2 && active && e < lp.nout;
-        outm[k2] = ok ? lp.inv[zline_cell(lp, pt) + e] : -1;
+        outm[k2] = ok ? lp.inv[zline_cell(lp, zl) + e] : -1;
       }
     }
     const V twv = load_or_zero(tid < M, reinterpret_cast<const V*>(lp.tw) + tid);
@@ -784,7 +808,7 @@ __global__ void __launch_bounds__((reg2_max_threads<V, LB, MODE>())) k_reg2(Line
 #pragma unroll
           for (int k2 = 0; k2 < R2; ++k2) {
             const int e = k1 + R1 * k2;
-            const int m = is_gather<MODE>() ? outm[k2] : (e < lp.nout ? (int)zline_cell(lp, pt) + e : -1);
+            const int m = is_gather<MODE>() ? outm[k2] : (e < lp.nout ? (int)zline_cell(lp, zl) + e : -1);
             row[k2] = m >= 0 ? 3 * m + c : -1;
           }
           if (is_gather<MODE>()) {
@@ -821,13 +845,14 @@ __global__ void __launch_bounds__((reg2_max_threads<V, LB, MODE>())) k_reg2(Line
         }
       } else {
         V* __restrict__ dst = reinterpret_cast<V*>(lp.dst) + (long long)lo * lp.lo_dst;
-        const int ga = pt / lp.ptb, gb = pt % lp.ptb;
+        const int ga = zl / lp.ptb, gb = zl % lp.ptb;
         const int de = (int)lp.de;
         V* __restrict__ obase = dst + ga * lp.da + gb * lp.db + q + k1 * de;
 #pragma unroll
         for (int k2 = 0; k2 < R2; ++k2) {
           const int e = k1 + R1 * k2;
-          if (e < lp.nout) obase[k2 * (R1 * de)] = a[k2];
+          const bool held = MODE != PASS_PLAIN_INV || t1_line_held(lp, ga, e);  // y inverse writes T1
+          if (e < lp.nout && held) obase[k2 * (R1 * de)] = a[k2];
         }
       }
     }

@@ -70,6 +70,12 @@ struct LinePass {
   void* dst;
   // scatter / gather (solver layout)
   const int* inv;       // cell -> occupied index or -1
+  // occupied blocks: the z lines (ix * Ny + iy) holding an occupied cell.
+  // The z passes run only those (point pt -> line zlines[pt]); the y inverse
+  // pass writes T1 only on those (lmask, one byte per line), so T1 keeps the
+  // zeros its other lines were cleared to.  Null: all.
+  const int* zlines;
+  const unsigned char* lmask;
   int nyz_y, nz;        // cuboid Ny, Nz (z-line points are (ix, iy))
   int L, Lb;            // total columns, column tile
   int nlo;              // number of column tiles (grid.y)

@@ -97,14 +97,28 @@ struct bdda_plan {
   // occupied extent of the target: every occupied cell has ix < occ[0],
   // iy < occ[1], iz < occ[2] (the cuboid's dims until a target is set)
   int occ[3] = {0, 0, 0};
+  // the z lines (ix * Ny + iy) that hold an occupied cell, ascending, and a
+  // byte per line of the cuboid marking them
+  bdda::DevBuf zlines, lmask;
+  int nzl = 0;
   // twiddles
   bdda::DevBuf tw64[3], tw32[3];
   // Green half spectrum
   bdda::DevBuf g64, g32;
   // target
   bdda::DevBuf inv, pos, diag64, jpre64;
-  // FFT intermediates
-  bdda::DevBuf T1, T2;
+  // FFT intermediates (T1 per transform type: complex128, complex64)
+  bdda::DevBuf T1, T1f, T2;
+  // For the occupied operator, T1 holds zeros on every z line outside the
+  // occupied-line list (the y forward pass reads them; nothing writes them).
+  // t1_zero[type] records the layout that invariant holds for.
+  struct T1Zero {
+    void* ptr = nullptr;
+    int Q = 0, nlo = 0;
+    unsigned long long lines = 0;
+    bool operator==(const T1Zero& o) const { return ptr == o.ptr && Q == o.Q && nlo == o.nlo && lines == o.lines; }
+  } t1_zero[2];
+  unsigned long long lines_version = 0;
   // solver workspace (blocks of 3N x L)
   bdda::DevBuf blk[12];
   bdda::DevBuf gram_part, gram_out, coef, dots_d, dots_n, obs_part, Bj64;
@@ -196,7 +210,8 @@ static int ensure_fft_ws(bdda_plan* p, int L) {
   const long long nlo = (L + Lb - 1) / Lb;
   const long long Q = 3LL * Lb;
   const long long nx = p->ax[0].n, ny = p->ax[1].n, mz = p->ax[2].m, my = p->ax[1].m;
-  int rc = p->T1.ensure((size_t)(nlo * nx * ny * mz * Q) * sizeof(V));
+  bdda::DevBuf& T1 = sizeof(V) == sizeof(double2) ? p->T1 : p->T1f;
+  int rc = T1.ensure((size_t)(nlo * nx * ny * mz * Q) * sizeof(V));
   if (rc) return rc;
   return p->T2.ensure((size_t)(nlo * nx * my * mz * Q) * sizeof(V));
 }
@@ -234,6 +249,25 @@ static int apply_operator(bdda_plan* p, const BV* S, BV* AS, int L, bool occupie
   g_err.clear();
   const long long t1_lo = (long long)nx * ny * mz * Q;
   const long long t2_lo = (long long)nx * my * mz * Q;
+  bdda::DevBuf& T1 = sizeof(V) == sizeof(double2) ? p->T1 : p->T1f;
+  {
+    // zeros on the z lines the occupied operator skips (once per layout);
+    // the cuboid passes write every line
+    auto& z = p->t1_zero[sizeof(V) == sizeof(double2) ? 0 : 1];
+    if (occupied) {
+      bdda_plan::T1Zero key;
+      key.ptr = T1.p;
+      key.Q = Q;
+      key.nlo = nlo;
+      key.lines = p->lines_version;
+      if (!(z == key)) {
+        BDDA_HIP(hipMemsetAsync(T1.p, 0, (size_t)(nlo * t1_lo) * sizeof(V), s));
+        z = key;
+      }
+    } else {
+      z = bdda_plan::T1Zero();
+    }
+  }
   LinePass base;
   std::memset(&base, 0, sizeof(base));
   base.Q = Q;
@@ -267,13 +301,17 @@ static int apply_operator(bdda_plan* p, const BV* S, BV* AS, int L, bool occupie
     lp.mode = occupied ? (wide ? PASS_SCATTER_Z64 : PASS_SCATTER_Z) : PASS_CUBOID_Z;
     lp.nin = oz;
     lp.nout = mz;
-    lp.npts = ox * oy;
-    lp.ptb = oy;
+    lp.npts = nx * ny;  // line pt = ix * Ny + iy
+    lp.ptb = ny;
+    if (occupied) {  // only the lines that hold an occupied cell
+      lp.zlines = p->zlines.as<int>();
+      lp.npts = p->nzl;
+    }
     lp.da = (long long)ny * mz * Q;
     lp.db = (long long)mz * Q;
     lp.de = Q;
     lp.lo_dst = t1_lo;
-    lp.dst = p->T1.p;
+    lp.dst = T1.p;
     lp.order_pe = 0;
     launch_timed<V>(p, lp, 0, s);
     BDDA_CHECK_LAUNCH();
@@ -295,7 +333,7 @@ static int apply_operator(bdda_plan* p, const BV* S, BV* AS, int L, bool occupie
     lp.db = Q;
     lp.de = (long long)mz * Q;
     lp.lo_dst = t2_lo;
-    lp.src = p->T1.p;
+    lp.src = T1.p;
     lp.dst = p->T2.p;
     lp.order_pe = 1;
     launch_timed<V>(p, lp, 1, s);
@@ -331,6 +369,7 @@ static int apply_operator(bdda_plan* p, const BV* S, BV* AS, int L, bool occupie
     lp.nin = my;
     lp.nout = oy;
     lp.npts = ox * mz;
+    if (occupied) lp.lmask = p->lmask.as<unsigned char>();  // T1 keeps zeros off the occupied lines
     lp.ptb = mz;
     lp.sa = (long long)my * mz * Q;
     lp.sb = Q;
@@ -341,7 +380,7 @@ static int apply_operator(bdda_plan* p, const BV* S, BV* AS, int L, bool occupie
     lp.de = (long long)mz * Q;
     lp.lo_dst = t1_lo;
     lp.src = p->T2.p;
-    lp.dst = p->T1.p;
+    lp.dst = T1.p;
     lp.order_pe = 1;
     launch_timed<V>(p, lp, 3, s);
     BDDA_CHECK_LAUNCH();
@@ -354,13 +393,17 @@ static int apply_operator(bdda_plan* p, const BV* S, BV* AS, int L, bool occupie
     lp.inverse = 1;
     lp.nin = mz;
     lp.nout = oz;
-    lp.npts = ox * oy;
-    lp.ptb = oy;
+    lp.npts = nx * ny;
+    lp.ptb = ny;
+    if (occupied) {
+      lp.zlines = p->zlines.as<int>();
+      lp.npts = p->nzl;
+    }
     lp.sa = (long long)ny * mz * Q;
     lp.sb = (long long)mz * Q;
     lp.se = Q;
     lp.lo_src = t1_lo;
-    lp.src = p->T1.p;
+    lp.src = T1.p;
     lp.scale = 1.0 / ((double)mx * my * mz);
     lp.order_pe = 0;
     launch_timed<V>(p, lp, 4, s);
@@ -1020,7 +1063,21 @@ int bdda_set_target(bdda_plan* p, const int64_t* address, int64_t N, const doubl
     }
     for (int d = 0; d < 3; ++d) p->occ[d] = ext[d];
   }
-  int rc = p->inv.ensure(p->ncell * sizeof(int));
+  std::vector<int> lines;
+  std::vector<unsigned char> mask((size_t)p->ax[0].n * p->ax[1].n, 0);
+  for (int64_t i = 0; i < N; ++i) {
+    const int l = (int)(address[i] / p->ax[2].n);  // addresses ascend, so lines do too
+    if (lines.empty() || lines.back() != l) lines.push_back(l);
+    mask[l] = 1;
+  }
+  int rc = p->zlines.ensure(lines.size() * sizeof(int));
+  if (!rc) rc = p->lmask.ensure(mask.size());
+  if (rc) return rc;
+  BDDA_HIP(hipMemcpy(p->zlines.p, lines.data(), lines.size() * sizeof(int), hipMemcpyHostToDevice));
+  BDDA_HIP(hipMemcpy(p->lmask.p, mask.data(), mask.size(), hipMemcpyHostToDevice));
+  p->nzl = (int)lines.size();
+  ++p->lines_version;
+  rc = p->inv.ensure(p->ncell * sizeof(int));
   if (!rc) rc = p->pos.ensure(N * 3 * sizeof(double));
   if (!rc) rc = p->diag64.ensure(3 * N * sizeof(double2));
   if (!rc) rc = p->jpre64.ensure(3 * N * sizeof(double2));

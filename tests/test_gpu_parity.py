@@ -195,6 +195,37 @@ def test_block_mvp_on_a_confined_target_matches_oracle(n, L, box):
     assert pl.occupied_extent == box
 
 
+def test_block_mvp_after_other_products_on_the_same_plan():
+    """The occupied operator keeps zeros on T1's unoccupied z lines and
+    skips them; cuboid products (every line written), the complex64
+    transform and another column count on the same plan in between must not
+    leave it reading stale lines."""
+    say("block mvp interleaved with cuboid products")
+    from bl_krylov import bl_krylov as pk
+    from mvp_fft import mvp_fft as pm
+    n, box = (20, 18, 16), (9, 6, 16)
+    lf, k = 0.03, 9.5
+    rng = np.random.default_rng(23)
+    cells = np.array([(ix * n[1] + iy) * n[2] + iz for ix in range(box[0]) for iy in range(box[1])
+                      for iz in range(box[2])])
+    address = np.sort(rng.choice(cells, size=cells.size // 4, replace=False)).astype(np.int64)
+    N = address.size
+    diag = rng.standard_normal(3 * N) + 1j * rng.standard_normal(3 * N) + 3.0
+    jpre = 1.0 / diag
+    idx = pk._build_scatter_index(address, 3)
+    At = pm.fft_init(n, 3, lf, k)
+    Ao = omvp.fft_init(n, 3, lf, k)
+    ncell = int(np.prod(n))
+    for L in (8, 5, 8):
+        S = rng.standard_normal((3 * N, L)) + 1j * rng.standard_normal((3 * N, L))
+        ref = okr.block_mvp(n, 3, Ao, idx, diag, S, jpre)
+        assert rel(pk._block_mvp(n, 3, At, idx, diag, S, jpre), ref) < 1e-12
+        P = rng.standard_normal((3 * ncell, L)) + 1j * rng.standard_normal((3 * ncell, L))
+        pm.fft_mvp(n, 3, At, P)
+        pm.fft_mvp(n, 3, At, P, precision="mixed")
+        assert rel(pk._block_mvp(n, 3, At, idx, diag, S, jpre), ref) < 1e-12
+
+
 def test_column_tile_prefers_the_compiled_8_column_tile():
     """Among equally padded tiles the operator takes 8 columns (the register
     kernels' compile-time tile); a tile that pads L less wins otherwise."""
