@@ -89,6 +89,10 @@ int bdda_mvp_offdiag(bdda_plan* plan, const void* P_dev, void* Y_dev, int L, int
  * pos_host: (N, 3) float64 positions (NULL: no positions; bdda_rhs and
  * bdda_observables then refuse); diag_host, jpre_host: (3N) complex128
  * diag(A) and 1/diag(A).
+ * The plan also records the occupied extent (bounds of the occupied cells)
+ * and the z lines holding an occupied cell: the occupied-block operator
+ * (bdda_block_mvp, the solvers) skips every line and row outside them,
+ * which hold only zeros.
  * Replaces _build_scatter_index and the jpre set-up of the solvers
  * (bl_krylov/bl_krylov.py:5-24,84,146). */
 int bdda_set_target(bdda_plan* plan, const int64_t* address_host, int64_t N, const double* pos_host,
