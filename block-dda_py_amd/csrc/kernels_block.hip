@@ -5,6 +5,7 @@
 // L orientation columns innermost, so a wavefront reads 64 consecutive
 // columns of one row.  Small L x L coefficient matrices are complex float64.
 #include <mutex>
+#include <type_traits>
 
 #include "internal.h"
 
@@ -108,6 +109,88 @@ __global__ void __launch_bounds__(256) k_gram(const V* __restrict__ A, const V* 
     for (int j = 0; j < 4; ++j) out[(ty + TQ * i) * TL + tx + TQ * j] = acc[i][j];
 }
 
+// The TL x TL tile (TL = 64 or 32) on the float64 matrix cores (complex128
+// blocks): wave w of the block computes tile rows (TL/2) (w / 2) ..
+// +TL/2 and columns (TL/2) (w % 2) .. +TL/2 as NT x NT products of 16 x 16
+// (NT = TL / 32), with the real and the imaginary part in separate
+// accumulators.  One v_mfma_f64_16x16x4f64 takes four rows of A and B: lane
+// l holds row k0 + l / 16 and column l % 16 of a 16-column strip of each,
+// loaded straight from the row-major blocks (a strip of one row is 256
+// contiguous bytes); result entry r of lane l is row l / 16 + 4 r, column
+// l % 16 of the 16 x 16 product (checked on the device against a host
+// product).  The next four rows are loaded while the current ones multiply.
+// Partial layout as k_gram with one group.
+typedef double gram_d4 __attribute__((ext_vector_type(4)));
+// Row `row` of this lane's NT strips of A and B (zero beyond r1 or L).
+template <int NT>
+__device__ __forceinline__ void gram_fetch(const double2* __restrict__ A, const double2* __restrict__ B, long long row,
+                                           long long r1, int L, int ja, int lb, double2 (&a)[NT], double2 (&b)[NT]) {
+  const bool ok = row < r1;
+#pragma unroll
+  for (int i = 0; i < NT; ++i) {
+    a[i] = (ok && ja + 16 * i < L) ? A[row * L + ja + 16 * i] : make_double2(0.0, 0.0);
+    b[i] = (ok && lb + 16 * i < L) ? B[row * L + lb + 16 * i] : make_double2(0.0, 0.0);
+  }
+}
+template <bool CONJ, int TL>
+__global__ void __launch_bounds__(256) k_gram_mfma(const double2* __restrict__ A, const double2* __restrict__ B,
+                                                   long long rows, int L, double2* __restrict__ partial) {
+  constexpr int WS = TL / 2, NT = WS / 16;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int kr = lane >> 4, cc = lane & 15;
+  const int ntile = (L + TL - 1) / TL;
+  const int jt = blockIdx.y / ntile, lt = blockIdx.y % ntile;
+  const long long chunk = (rows + gridDim.x - 1) / gridDim.x;
+  const long long r0 = blockIdx.x * chunk;
+  const long long r1 = min(rows, r0 + chunk);
+  const int ja = jt * TL + WS * (w >> 1) + cc, lb = lt * TL + WS * (w & 1) + cc;
+  gram_d4 cr[NT][NT], ci[NT][NT];
+#pragma unroll
+  for (int i = 0; i < NT; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      cr[i][j] = gram_d4{0.0, 0.0, 0.0, 0.0};
+      ci[i][j] = gram_d4{0.0, 0.0, 0.0, 0.0};
+    }
+  double2 a[NT], b[NT];
+  if (r0 < r1) gram_fetch<NT>(A, B, r0 + kr, r1, L, ja, lb, a, b);
+  for (long long k0 = r0; k0 < r1; k0 += 4) {
+    double2 ca[NT], cb[NT];
+#pragma unroll
+    for (int i = 0; i < NT; ++i) {
+      ca[i] = a[i];
+      cb[i] = b[i];
+    }
+    if (k0 + 4 < r1) gram_fetch<NT>(A, B, k0 + 4 + kr, r1, L, ja, lb, a, b);
+    // conj:  Re += ar br + ai bi, Im += ar bi - ai br
+    // plain: Re += ar br - ai bi, Im += ar bi + ai br
+#pragma unroll
+    for (int i = 0; i < NT; ++i)
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[i].x, cb[j].x, cr[i][j], 0, 0, 0);
+        ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[i].x, cb[j].y, ci[i][j], 0, 0, 0);
+      }
+#pragma unroll
+    for (int i = 0; i < NT; ++i)
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(CONJ ? ca[i].y : -ca[i].y, cb[j].y, cr[i][j], 0, 0, 0);
+        ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(CONJ ? -ca[i].y : ca[i].y, cb[j].x, ci[i][j], 0, 0, 0);
+      }
+  }
+  double2* out = partial + ((long long)blockIdx.x * gridDim.y + blockIdx.y) * (TL * TL);
+#pragma unroll
+  for (int i = 0; i < NT; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int jl = WS * (w >> 1) + 16 * i + kr + 4 * r, ll = WS * (w & 1) + 16 * j + cc;
+        out[jl * TL + ll] = make_double2(cr[i][j][r], ci[i][j][r]);
+      }
+}
+
 // out[j][l] = sum of the parts' TL x TL tiles.  A block of 256 threads takes
 // E consecutive tile entries and G = 256 / E part groups (E chosen by the
 // launcher so that each thread's chain over parts stays short): group g sums
@@ -142,13 +225,24 @@ static int block_tile(int L) { return L <= 16 ? 16 : (L <= 32 ? 32 : 64); }
 template <typename V, int TL>
 static void launch_gram_tile(const V* A, const V* B, long long rows, int L, bool conjA, double2* partial,
                              int nparts, double2* out, hipStream_t s) {
-  constexpr int NG = 256 / ((TL / 4) * (TL / 4));
+  // complex128 blocks with L > 32: the matrix-core kernel (one group).  On
+  // the 32-column tile it measured slower than the vector kernel (that Gram
+  // is bound by HBM reads, and the vector kernel's four row groups keep
+  // more of them in flight).
+  constexpr bool mfma = std::is_same<V, double2>::value && TL == 64;
+  constexpr int NG = mfma ? 1 : 256 / ((TL / 4) * (TL / 4));
   const int ntile = (L + TL - 1) / TL;
   dim3 grid(nparts, ntile * ntile);
-  if (conjA)
+  if constexpr (mfma) {
+    if (conjA)
+      hipLaunchKernelGGL((k_gram_mfma<true, TL>), grid, dim3(256), 0, s, A, B, rows, L, partial);
+    else
+      hipLaunchKernelGGL((k_gram_mfma<false, TL>), grid, dim3(256), 0, s, A, B, rows, L, partial);
+  } else if (conjA) {
     hipLaunchKernelGGL((k_gram<V, true, TL>), grid, dim3(256), 0, s, A, B, rows, L, partial);
-  else
+  } else {
     hipLaunchKernelGGL((k_gram<V, false, TL>), grid, dim3(256), 0, s, A, B, rows, L, partial);
+  }
   // reduction block shape: about 32 parts per thread's chain
   const int parts = nparts * NG;
   int G = 1;
