@@ -368,10 +368,97 @@ __global__ void __launch_bounds__(256) k_update(V* Y, const V* X, double2 a, con
   }
 }
 
+// The same update on the float64 matrix cores, for complex128 blocks with
+// 32 < L <= 64 (one 64-column tile, K = L <= 64).  C (L x 64) is staged in
+// LDS once per block; the block then walks slabs of 64 rows, wave w taking
+// rows 16 w .. +16 against all 64 columns.  Step s takes coefficients
+// j = 4 s .. 4 s + 3: lane l loads row l % 16, coefficient 4 s + l / 16 of
+// W1 + w2c W2 and runs four 16 x 16 column tiles through
+// v_mfma_f64_16x16x4f64, real and imaginary parts in separate accumulators.  Entry r of lane l of a tile is row l / 16 + 4 r,
+// column l % 16; the epilogue adds a X and e E there.  Sums over j run in a
+// fixed order (the hardware's within each step of four), so repeated
+// updates agree bit for bit.
+template <int KS>
+__global__ void __launch_bounds__(256) k_update_mfma(double2* Y, const double2* X, double2 a, const double2* W1,
+                                                     const double2* W2, double2 w2c,
+                                                     const double2* __restrict__ C, const double2* E, double2 e,
+                                                     long long rows, int L) {
+  __shared__ double2 Cs[4 * KS][64];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int kr = lane >> 4, cc = lane & 15;
+  for (int idx = tid; idx < 4 * KS * 64; idx += 256) {
+    const int j = idx / 64, l = idx % 64;
+    Cs[j][l] = (j < L && l < L) ? C[(long long)j * L + l] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  for (long long r0 = blockIdx.x * 64LL; r0 < rows; r0 += 64LL * gridDim.x) {
+    const long long row = r0 + 16 * w + cc;  // this lane's W row
+    gram_d4 yr[4], yi[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      yr[t] = gram_d4{0.0, 0.0, 0.0, 0.0};
+      yi[t] = gram_d4{0.0, 0.0, 0.0, 0.0};
+    }
+#pragma unroll 4
+    for (int st = 0; st < KS; ++st) {
+      const int j = 4 * st + kr;
+      double2 v = make_double2(0.0, 0.0);
+      if (row < rows && j < L) {
+        v = W1[row * L + j];
+        if (W2) {
+          const double2 u = W2[row * L + j];
+          v.x += w2c.x * u.x - w2c.y * u.y;
+          v.y += w2c.x * u.y + w2c.y * u.x;
+        }
+      }
+      const double2 ct[4] = {Cs[j][cc], Cs[j][16 + cc], Cs[j][32 + cc], Cs[j][48 + cc]};
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        yr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(v.x, ct[t].x, yr[t], 0, 0, 0);
+        yi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(v.x, ct[t].y, yi[t], 0, 0, 0);
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        yr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-v.y, ct[t].y, yr[t], 0, 0, 0);
+        yi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(v.y, ct[t].x, yi[t], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long long orow = r0 + 16 * w + kr + 4 * r;
+        const int l = 16 * t + cc;
+        if (orow >= rows || l >= L) continue;
+        const long long o = orow * L + l;
+        double2 y = make_double2(yr[t][r], yi[t][r]);
+        if (X) {
+          const double2 x = X[o];
+          y.x += a.x * x.x - a.y * x.y;
+          y.y += a.x * x.y + a.y * x.x;
+        }
+        if (E) {
+          const double2 ev = E[o];
+          y.x += e.x * ev.x - e.y * ev.y;
+          y.y += e.x * ev.y + e.y * ev.x;
+        }
+        Y[o] = y;
+      }
+  }
+}
+
 template <typename V, int TL>
 static void launch_update_tile(V* Y, const V* X, double2 a, const V* W1, const V* W2, double2 w2c,
                                const double2* C, const V* E, double2 e, long long rows, int L, hipStream_t s) {
   constexpr int RB = 4096 / TL;
+  if constexpr (std::is_same<V, double2>::value && TL == 64) {
+    if (L <= 64) {  // one column tile: the matrix-core kernel
+      const long long slabs = (rows + 63) / 64;
+      const int g = (int)min(slabs, 1024LL);
+      hipLaunchKernelGGL((k_update_mfma<16>), dim3(g), dim3(256), 0, s, Y, X, a, W1, W2, w2c, C, E, e, rows, L);
+      return;
+    }
+  }
   const long long slabs = (rows + RB - 1) / RB;
   const int g = (int)min(slabs, 4096LL);
   dim3 grid(g, (L + TL - 1) / TL);

@@ -307,10 +307,12 @@ def _gpu_dd(prob, solver="bicgstab", tol=1e-2, itermax=25, precision="fp64", **k
 
 
 @pytest.mark.parametrize("solver", ["cocg_rq", "bicgstab"])
-@pytest.mark.parametrize("L", [6, 24, 70])
+@pytest.mark.parametrize("L", [6, 24, 37, 64, 70])
 def test_solver_iterates_and_observables_match_oracle(solver, L):
     """Block solves against the oracle; L = 6, 24 and 70 put the Gram and
-    update kernels on their 16-, 32- and 64-wide tiles (two of them at 70)."""
+    update kernels on their 16-, 32- and 64-wide tiles (two of them at 70);
+    37 and 64 take the matrix-core Gram and update on one 64-wide tile (37:
+    partial coefficient steps and columns)."""
     say(f"solver {solver} L={L}")
     prob = _problem(L=L)
     ref = _oracle_solve(prob, "cocg" if solver == "cocg_rq" else "bicgstab", 1e-8, 60)
