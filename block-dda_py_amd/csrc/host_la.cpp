@@ -52,7 +52,9 @@ inline void split_axpy_neg(double* __restrict yr, double* __restrict yi, double 
 #define BDDA_HOST_CLONES __attribute__((target_clones("avx512f", "arch=haswell", "default")))
 
 // In-place LU with partial pivoting (row interchanges recorded in piv);
-// returns false if exactly singular.
+// returns false if exactly singular.  The pivot is the first entry of
+// largest |Re| + |Im| in the column, LAPACK's choice (zgetrf through
+// izamax), which np.linalg.solve in the reference makes.
 BDDA_HOST_CLONES bool lu_factor(std::vector<zc>& A, std::vector<int>& piv, int n) {
   piv.resize(n);
   Split S;
@@ -61,9 +63,9 @@ BDDA_HOST_CLONES bool lu_factor(std::vector<zc>& A, std::vector<int>& piv, int n
   double* ai = S.im.data();
   for (int k = 0; k < n; ++k) {
     int p = k;
-    double best = std::hypot(ar[k * n + k], ai[k * n + k]);
+    double best = std::fabs(ar[k * n + k]) + std::fabs(ai[k * n + k]);
     for (int i = k + 1; i < n; ++i) {
-      const double v = std::hypot(ar[i * n + k], ai[i * n + k]);
+      const double v = std::fabs(ar[i * n + k]) + std::fabs(ai[i * n + k]);
       if (v > best) {
         best = v;
         p = i;
