@@ -5,6 +5,8 @@ geometry is built by the drop-in GRE model (host surface sampling, GPU
 voxeliser) and the orientations follow the reference's draw order
 (``run_dda.py:46-54``: alpha, beta, gamma uniform from one generator).
 """
+import os
+
 import numpy as np
 
 # 64^3 cuboid: GRE bc = 0.8, ab = 0.7, beta = 0.2 (seed 0).  r_v gives a
@@ -82,11 +84,57 @@ def euler_angles(L, seed):
     return np.column_stack([rng.uniform(0, 2 * np.pi, L), rng.uniform(0, np.pi, L), rng.uniform(0, 2 * np.pi, L)])
 
 
-def build_geometry(wl):
-    """(name, lattice_n, lf, grid_points, is_in) via the GPU voxeliser."""
+# Coarse GRE log-radius fields of the workloads above, drawn once on an
+# x86-64 host with OpenBLAS (see record_fields).  The draw goes through
+# LAPACK's eigendecomposition of the covariance, whose eigenvector signs
+# depend on the BLAS kernel the CPU selects, so the same seed gives another
+# particle on another host; the recorded fields make every host build the
+# same particles.
+FIELDS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "gre_fields.npz")
+_fields = None
+
+
+def recorded_field(wl):
+    """The recorded 25 x 100 coarse field of workload `wl`, or None."""
+    global _fields
+    if _fields is None:
+        _fields = {}
+        if os.path.exists(FIELDS_PATH):
+            with np.load(FIELDS_PATH) as d:
+                _fields = {k: d[k] for k in d.files}
+    return _fields.get(wl["name"])
+
+
+def all_workloads():
+    """Every GRE particle the workloads use (the batch expanded)."""
+    return [SMALL_24, FULL_64, LARGE_96] + batch_instances(BATCH_32)
+
+
+def record_fields(path=FIELDS_PATH):
+    """Draw and store the coarse field of every workload particle
+    (gaussian_ellipsoid.py:106-120: one multivariate_normal draw with the
+    workload's shape seed)."""
+    from shape_model.gaussian_ellipsoid import gaussian_ellipsoid_shape_model as G
+    out = {}
+    for wl in all_workloads():
+        g = G(wl["r_v_base"], wl["bc_ratio"], wl["ab_ratio"], wl["beta"])
+        out[wl["name"]] = g.draw_field(np.random.default_rng(wl["shape_seed"]))
+    np.savez_compressed(path, **out)
+    return path
+
+
+def build_geometry(wl, recorded=True):
+    """(name, lattice_n, lf, grid_points, is_in) via the GPU voxeliser.
+    With `recorded`, a workload whose coarse field is recorded is built from
+    it; otherwise (or if none is) the field is drawn from its shape seed."""
     from shape_model.gaussian_ellipsoid import gaussian_ellipsoid_shape_model as G
     g = G(wl["r_v_base"], wl["bc_ratio"], wl["ab_ratio"], wl["beta"])
-    pts, _ = g.compute_r_points_on_GRE(np.random.default_rng(wl["shape_seed"]))
+    s = recorded_field(wl) if recorded else None
+    if s is None:
+        pts, _ = g.compute_r_points_on_GRE(np.random.default_rng(wl["shape_seed"]))
+    else:
+        g.set_name()
+        pts, _ = g.surface_from_sample(s)
     _, n, grid = g.create_cuboid_lattice_that_encloses_GRE_shape(pts)
     dist = g.find_nearest_distance_from_the_GRE_surf(grid, pts)
     is_in = g.extract_lattice_address_in_GRE_volume(g.lattice_lf, g.distance_factor, n, dist)

@@ -43,12 +43,22 @@ class gaussian_ellipsoid_shape_model:  # noqa: N801  (reference class name)
 
     def compute_r_points_on_GRE(self, rng):
         """Surface points and meshes of the deformed ellipsoid (gaussian_ellipsoid.py:73-147)."""
+        self.set_name()
+        return self.surface_from_sample(self.draw_field(rng))
+
+    def set_name(self):
+        """The particle's name string (gaussian_ellipsoid.py:86-88)."""
         self.name = ("GRE_shape__r_v_base={:.2f}um__ab_ratio={:.1f}"
                      "__bc_ratio={:.1f}__beta={:.2f}".format(self.r_v_base, self.ab_ratio, self.bc_ratio,
                                                             self.beta))
+
+    def draw_field(self, rng):
+        """One draw of the coarse log-radius field, N_THETA x N_PHI
+        (gaussian_ellipsoid.py:106-120): covariance beta^2 exp(-d^2/(2 lc^2)),
+        multivariate_normal with method='eigh'."""
         cov = self.beta ** 2 * self.correlation_matrix()
         draw = rng.multivariate_normal(np.zeros(self.N_THETA * self.N_PHI), cov, size=1, method='eigh')
-        return self.surface_from_sample(draw.squeeze().reshape(self.N_THETA, self.N_PHI))
+        return draw.squeeze().reshape(self.N_THETA, self.N_PHI)
 
     def correlation_matrix(self):
         """exp(-d^2 / (2 lc^2)) on the coarse grid; independent of r_v and beta

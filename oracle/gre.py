@@ -44,37 +44,45 @@ class GREShape:
         a = b * self.ab_ratio
         return a, b, c, c ** 2 / a, c * 0.3
 
-    def surface_points(self, rng):
-        """Deformed surface points, ``gaussian_ellipsoid.py:73-147``.
-
-        Covariance beta^2 exp(-d^2/(2 lc^2)) on a 25x100 (theta, phi) grid with
-        component-wise squared differences (``:108-113``), one
-        ``multivariate_normal(method='eigh')`` draw (``:115-117``), bicubic
-        spline upsampling by int(4 cbrt(bc*ab)) (``:122-130``) and normal
-        displacement h0((e^s - beta^2/2) - 1) (``:136-142``).
-        Returns (P, 3) points, P = (25 f) * (100 f).
-        """
-        self.name = ("GRE_shape__r_v_base={:.2f}um__ab_ratio={:.1f}"
-                     "__bc_ratio={:.1f}__beta={:.2f}".format(
-                         self.r_v_base, self.ab_ratio, self.bc_ratio, self.beta))
-        a, b, c, h0, lc = self.semi_axes()
-        n_t, n_p = 25, 100
-        th = np.linspace(0, np.pi, n_t)
-        ph = np.linspace(0, 2 * np.pi, n_p)
+    def coarse_grid(self):
+        """Coarse (theta, phi) grid and its ellipsoid points, ``gaussian_ellipsoid.py:96-104``."""
+        a, b, c, _, _ = self.semi_axes()
+        th = np.linspace(0, np.pi, 25)
+        ph = np.linspace(0, 2 * np.pi, 100)
         thm, phm = np.meshgrid(th, ph, indexing='ij')
         px = (a * np.sin(thm) * np.cos(phm)).ravel()
         py = (b * np.sin(thm) * np.sin(phm)).ravel()
         pz = (c * np.cos(thm)).ravel()
+        return th, ph, px, py, pz
+
+    def covariance(self):
+        """beta^2 exp(-d^2 / (2 lc^2)) on the coarse grid with component-wise
+        squared differences summed x, then y, then z (``:108-113``)."""
+        _, _, _, _, lc = self.semi_axes()
+        _, _, px, py, pz = self.coarse_grid()
         d2 = ((px[:, None] - px[None, :]) ** 2
               + (py[:, None] - py[None, :]) ** 2
               + (pz[:, None] - pz[None, :]) ** 2)
-        cov = self.beta ** 2 * np.exp(-0.5 * d2 / lc ** 2)
-        s = rng.multivariate_normal(np.zeros(n_t * n_p), cov, size=1,
-                                    method='eigh').squeeze().reshape(n_t, n_p)
+        return self.beta ** 2 * np.exp(-0.5 * d2 / lc ** 2)
+
+    def coarse_field(self, rng):
+        """One ``multivariate_normal(method='eigh')`` draw of the log-radius
+        field on the 25 x 100 grid (``:115-120``)."""
+        cov = self.covariance()
+        return rng.multivariate_normal(np.zeros(cov.shape[0]), cov, size=1,
+                                       method='eigh').squeeze().reshape(25, 100)
+
+    def surface_from_field(self, s):
+        """Deformed surface points for a coarse field s (25 x 100): bicubic
+        spline upsampling by int(4 cbrt(bc*ab)) (``:122-130``) and normal
+        displacement h0((e^s - beta^2/2) - 1) (``:132-147``).
+        Returns (P, 3) points, P = (25 f) * (100 f)."""
+        a, b, c, h0, _ = self.semi_axes()
+        th, ph, _, _, _ = self.coarse_grid()
         spline = scipy.interpolate.RectBivariateSpline(th, ph, s)
         fac = int(4 * np.cbrt(self.bc_ratio * self.ab_ratio))
-        th2 = np.linspace(0, np.pi, fac * n_t)
-        ph2 = np.linspace(0, 2 * np.pi, fac * n_p)
+        th2 = np.linspace(0, np.pi, fac * 25)
+        ph2 = np.linspace(0, 2 * np.pi, fac * 100)
         thm2, phm2 = np.meshgrid(th2, ph2, indexing='ij')
         s2 = spline(th2, ph2)
         x0 = a * np.sin(thm2) * np.cos(phm2)
@@ -87,6 +95,14 @@ class GREShape:
         yg = y0 + deform * nvec[1]
         zg = z0 + deform * nvec[2]
         return np.column_stack([xg.ravel(), yg.ravel(), zg.ravel()])
+
+    def surface_points(self, rng):
+        """Deformed surface points, ``gaussian_ellipsoid.py:73-147``: the
+        coarse field draw, then the spline and the normal displacement."""
+        self.name = ("GRE_shape__r_v_base={:.2f}um__ab_ratio={:.1f}"
+                     "__bc_ratio={:.1f}__beta={:.2f}".format(
+                         self.r_v_base, self.ab_ratio, self.bc_ratio, self.beta))
+        return self.surface_from_field(self.coarse_field(rng))
 
     def cuboid_lattice(self, pts):
         """Enclosing cuboid lattice, ``gaussian_ellipsoid.py:150-179``.
