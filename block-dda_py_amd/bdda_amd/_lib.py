@@ -33,6 +33,7 @@ _SIGNATURES = {
     "bdda_plan_destroy": ([_vp], _i),
     "bdda_plan_fft_dims": ([_vp, _pi], _i),
     "bdda_green_init": ([_vp, _d, _d], _i),
+    "bdda_green_share": ([_vp, _vp], _i),
     "bdda_green_download": ([_vp, _vp], _i),
     "bdda_green_tensor": ([_vp, _d, _d, _vp], _i),
     "bdda_mvp_offdiag": ([_vp, _vp, _vp, _i, _i, _vp], _i),

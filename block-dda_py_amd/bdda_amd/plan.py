@@ -63,6 +63,12 @@ class Plan:
         check(self.lib.bdda_green_init(self.h, float(lf), float(k)), "green_init")
         self.lf, self.k = float(lf), float(k)
 
+    def share_green(self, other):
+        """Compute with `other`'s Green spectrum (same cuboid) instead of
+        building one."""
+        check(self.lib.bdda_green_share(self.h, other.h), "green_share")
+        self.lf, self.k = other.lf, other.k
+
     def green_spectrum(self):
         out = np.empty(self.m + (3, 3), dtype=np.complex128)
         check(self.lib.bdda_green_download(self.h, ptr(out)), "green_download")
@@ -134,7 +140,9 @@ class Plan:
 
     def solve(self, B, solver="bicgstab", precision="fp64", tol=1e-2, itermax=25, X0=None, refine_every=20,
               true_residual=True):
-        """Returns (X, iter_fin, err_fin, true_res, history, warm_start_used).
+        """Returns (X, iter_fin, err_fin, true_res, history, warm_start_used);
+        self.last_solve_info then holds the mixed-precision refinement
+        cycles and whether the solve fell back to complex128.
         A float64 solve stops on its recursive residual, as the reference's
         solvers do; true_residual=False skips the extra operator application
         that checks it (true_res is then None).  A mixed solve's err_fin is
@@ -154,6 +162,8 @@ class Plan:
                                   hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(flags),
                                   _stream()), "solve")
         nh = flags.value >> 8
+        self.last_solve_info = {"refine_cycles": (flags.value >> 2) & 63,
+                                "fallback_complex128": bool(flags.value & 2)}
         trv = tr.value if true_residual else (err.value if precision == "mixed" else None)
         return X, it.value, err.value, trv, hist[:nh].tolist(), bool(flags.value & 1)
 

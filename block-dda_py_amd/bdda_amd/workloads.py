@@ -10,8 +10,9 @@ import os
 import numpy as np
 
 # 64^3 cuboid: GRE bc = 0.8, ab = 0.7, beta = 0.2 (seed 0).  r_v gives a
-# 54 x 51 x 63 lattice (21,434 dipoles) padded with vacuum to 64^3, i.e. a
-# 128^3 doubled FFT grid; birefringent m_p; L = 64; BiCGSTAB to 1e-5.
+# 51 x 48 x 63 lattice (21,113 dipoles, from the recorded field) padded with
+# vacuum to 64^3, i.e. a 128^3 doubled FFT grid; birefringent m_p; L = 64;
+# BiCGSTAB to 1e-5.
 FULL_64 = dict(
     name="gre_bc0.8_ab0.7_beta0.2_seed0_64cube_L64",
     r_v_base=1.0107600709889086, bc_ratio=0.8, ab_ratio=0.7, beta=0.2, shape_seed=0,
@@ -31,7 +32,8 @@ SMALL_24 = dict(
 
 # Config-1 physics (bc = ab = 1, beta = 0.1, seed 0, m_p = 1.5 + 0.01i, m_m = 1)
 # scaled to a 96^3 cuboid at the same resolution: r_v = 6.2 um gives a
-# ~91 x 92 x 87 lattice (lf = 0.148 um) padded to 96^3 (192^3 FFT grid),
+# 91 x 92 x 87 lattice (307,817 dipoles, lf = 0.148 um) padded to 96^3
+# (192^3 FFT grid),
 # size parameter 12 (about 16 dipoles per wavelength, as config 1's 14);
 # L = 32; mixed-precision block-COCG-rq to 1e-5.
 LARGE_96 = dict(
@@ -150,14 +152,17 @@ def wavelength(wl, lf, n_occ):
     return 2 * np.pi * wl["m_m"] * r_ve / wl["size_parameter"]
 
 
-def make_dipoles(wl, geometry=None, precision="fp64", solver=None, verbose=False):
-    """DiscreteDipoles for a workload, interaction matrix and B set on the device."""
+def make_dipoles(wl, geometry=None, precision="fp64", solver=None, verbose=False, euler=None):
+    """DiscreteDipoles for a workload, interaction matrix and B set on the
+    device; `euler` (L, 3) replaces the workload's own orientations."""
     from bl_dda.scatterer import DiscreteDipoles, IncidentField, Target
     name, n, lf, grid, is_in = geometry if geometry is not None else build_geometry(wl)
     m_p = np.array(wl["m_p_xyz"], dtype=np.complex128)
     tg = Target(name, n, lf, grid, is_in, m_p)
     wl0 = wavelength(wl, lf, tg.num_element_occupy)
-    inc = IncidentField(wl0, wl["m_m"], euler_angles(wl["L"], wl["orient_seed"]))
+    if euler is None:
+        euler = euler_angles(wl["L"], wl["orient_seed"])
+    inc = IncidentField(wl0, wl["m_m"], euler)
     dd = DiscreteDipoles(tg, inc)
     dd.verbose = verbose
     dd.solver = solver or wl["solver"]

@@ -181,6 +181,7 @@ class DiscreteDipoles(Target, IncidentField):
         X, it, err, tr, hist, warm = self._plan.solve(self._B_dev, self.solver, self.precision, self.tol,
                                                       self.itermax, X0=self.X0, refine_every=self.refine_every,
                                                       true_residual=False)
+        self.solve_info = dict(self._plan.last_solve_info)
         self._store_solution(X, it, err, tr, hist, warm, time.time() - start)
 
     @property

@@ -103,8 +103,10 @@ struct bdda_plan {
   int nzl = 0;
   // twiddles
   bdda::DevBuf tw64[3], tw32[3];
-  // Green half spectrum
-  bdda::DevBuf g64, g32;
+  // Green half spectrum (complex128, complex64); shared between plans of
+  // the same cuboid, lattice spacing and wavenumber (bdda_green_share)
+  std::shared_ptr<bdda::DevBuf> g64, g32;
+  double green_lf = 0.0, green_k = 0.0;
   // target
   bdda::DevBuf inv, pos, diag64, jpre64;
   // FFT intermediates (T1 per transform type: complex128, complex64)
@@ -147,13 +149,13 @@ static int check_plan(const bdda_plan* p) {
 template <typename V> struct PrecTraits;
 template <> struct PrecTraits<double2> {
   static const void* tw(const bdda_plan* p, int a) { return p->tw64[a].p; }
-  static const void* green(const bdda_plan* p) { return p->g64.p; }
+  static const void* green(const bdda_plan* p) { return p->g64->p; }
   static const void* diag(const bdda_plan* p) { return p->diag64.p; }
   static const void* jpre(const bdda_plan* p) { return p->jpre64.p; }
 };
 template <> struct PrecTraits<float2> {
   static const void* tw(const bdda_plan* p, int a) { return p->tw32[a].p; }
-  static const void* green(const bdda_plan* p) { return p->g32.p; }
+  static const void* green(const bdda_plan* p) { return p->g32->p; }
   // occupied blocks are always complex128 (see apply_operator)
   static const void* diag(const bdda_plan*) { return nullptr; }
   static const void* jpre(const bdda_plan*) { return nullptr; }
@@ -899,10 +901,13 @@ int bdda_green_init(bdda_plan* p, double lf, double k) {
   const long long M = (long long)p->ax[0].m * p->ax[1].m * p->ax[2].m;
   const long long H = (long long)p->ax[0].h * p->ax[1].h * p->ax[2].h;
   DevBuf full;
+  // fresh buffers: a spectrum shared with other plans is never overwritten
+  auto g64 = std::make_shared<DevBuf>(), g32 = std::make_shared<DevBuf>();
   int rc = full.ensure((size_t)M * 6 * sizeof(double2));
-  if (!rc) rc = p->g64.ensure((size_t)H * 6 * sizeof(double2));
-  if (!rc) rc = p->g32.ensure((size_t)H * 6 * sizeof(float2));
+  if (!rc) rc = g64->ensure((size_t)H * 6 * sizeof(double2));
+  if (!rc) rc = g32->ensure((size_t)H * 6 * sizeof(float2));
   if (rc) return rc;
+  p->green_ready = false;
   hipStream_t s = 0;
   launch_green_build<double2>(full.as<double2>(), p->ax, lf, k, s);
   BDDA_HIP(hipGetLastError());
@@ -945,11 +950,37 @@ int bdda_green_init(bdda_plan* p, double lf, double k) {
     launch_line_pass<double2>(lp, s);
     BDDA_CHECK_LAUNCH();
   }
-  launch_green_halve<double2>(full.as<double2>(), p->g64.as<double2>(), p->ax, s);
-  launch_green_halve<float2>(full.as<double2>(), p->g32.as<float2>(), p->ax, s);
+  launch_green_halve<double2>(full.as<double2>(), g64->as<double2>(), p->ax, s);
+  launch_green_halve<float2>(full.as<double2>(), g32->as<float2>(), p->ax, s);
   BDDA_HIP(hipGetLastError());
   BDDA_HIP(hipDeviceSynchronize());
+  p->g64 = std::move(g64);
+  p->g32 = std::move(g32);
+  p->green_lf = lf;
+  p->green_k = k;
   p->green_ready = true;
+  return 0;
+}
+
+int bdda_green_share(bdda_plan* dst, const bdda_plan* src) {
+  if (int rc = check_plan(dst)) return rc;
+  if (int rc = check_plan(src)) return rc;
+  g_err.clear();
+  if (!src->green_ready) {
+    set_error("bdda_green_share: the source plan has no Green spectrum (call bdda_green_init on it)");
+    return BDDA_E_STATE;
+  }
+  for (int a = 0; a < 3; ++a)
+    if (dst->ax[a].n != src->ax[a].n || dst->ax[a].m != src->ax[a].m) {
+      set_error("bdda_green_share: the plans cover different cuboids");
+      return BDDA_E_ARG;
+    }
+  if (dst == src) return 0;
+  dst->g64 = src->g64;
+  dst->g32 = src->g32;
+  dst->green_lf = src->green_lf;
+  dst->green_k = src->green_k;
+  dst->green_ready = true;
   return 0;
 }
 
@@ -962,7 +993,7 @@ int bdda_green_download(const bdda_plan* p, void* out_host) {
   const long long M = (long long)p->ax[0].m * p->ax[1].m * p->ax[2].m;
   DevBuf full;
   if (int rc = full.ensure((size_t)M * 9 * sizeof(double2))) return rc;
-  launch_green_expand(p->g64.as<double2>(), full.as<double2>(), p->ax, nullptr, 0);
+  launch_green_expand(p->g64->as<double2>(), full.as<double2>(), p->ax, nullptr, 0);
   BDDA_HIP(hipGetLastError());
   BDDA_HIP(hipMemcpy(out_host, full.p, (size_t)M * 9 * sizeof(double2), hipMemcpyDeviceToHost));
   return 0;
@@ -1171,10 +1202,15 @@ int bdda_solve(bdda_plan* p, int solver, int precision, const void* B_dev, const
     // float64 true residual R = Bj - A X with the complex64 convolution;
     // every refine_every iterations it computes its true residual with the
     // complex128 operator, and once that exceeds twice the recursive one
-    // (complex64 rounding amplified by the recurrence) it stops.  After a
-    // cycle X += D and R is recomputed; a cycle that made R worse is undone.
-    // Any further cycle restarts from the true residual with the complex128
-    // convolution and the remaining iteration budget.
+    // (complex64 rounding amplified by the recurrence) it stops; it also
+    // stops when its recursive residual is off a log-linear course to the
+    // tolerance.  After a cycle X += D and R is recomputed; a cycle that made
+    // R worse is undone.  Classic iterative refinement: while complex64
+    // cycles reach their own tolerance and at least halve the true residual,
+    // the next cycle restarts from the new true residual in complex64 again;
+    // a cycle stopped by its check (a hard solve, where complex64 rounding
+    // costs the recurrence its conjugacy) or one that does not halve the
+    // residual hands the rest of the iteration budget to complex128.
     if ((rc = p->blk[9].ensure(b64)) || (rc = p->blk[10].ensure(b64))) return rc;
     double2* ws[7];
     for (int i = 0; i < 7; ++i) ws[i] = p->blk[i].as<double2>();
@@ -1182,6 +1218,7 @@ int bdda_solve(bdda_plan* p, int solver, int precision, const void* B_dev, const
     double2* D = p->blk[9].as<double2>();
     double2* Rsave = p->blk[10].as<double2>();
     bool conv32 = true;
+    int cycles = 0;
     if (x_zero) {
       BDDA_HIP(hipMemsetAsync(X, 0, b64, s));
       BDDA_HIP(hipMemcpyAsync(R64, Bj, b64, hipMemcpyDeviceToDevice, s));
@@ -1197,9 +1234,11 @@ int bdda_solve(bdda_plan* p, int solver, int precision, const void* B_dev, const
     err = res;
     while (res >= tol && total < itermax) {
       const double rs = std::sqrt(r2);
-      // the complex64 convolution's 1e-7 rounding bounds what a cycle reaches
+      // the complex64 convolution's 1e-7 rounding bounds what a complex64
+      // cycle reaches
       double inner_tol = 0.5 * tol * bnorm / rs;
-      if (inner_tol < 2e-6) inner_tol = 2e-6;
+      if (conv32 && inner_tol < 2e-6) inner_tol = 2e-6;
+      ++cycles;
       std::vector<double> inner;
       c.hist = &inner;
       c.conv32 = conv32;
@@ -1242,8 +1281,14 @@ int bdda_solve(bdda_plan* p, int solver, int precision, const void* B_dev, const
         BDDA_HIP(hipMemcpyAsync(R64, Rsave, b64, hipMemcpyDeviceToDevice, s));
         r2new = r2;
       }
-      // a further cycle restarts the Krylov space; it runs in complex128
-      conv32 = false;
+      // the next cycle restarts the Krylov space from the true residual: in
+      // complex64 again if this one was a complex64 cycle that reached its
+      // tolerance (not stopped by its check) and at least halved the
+      // residual, else in complex128
+      if (!(was32 && conv32 && r2new <= 0.25 * r2)) {
+        if (was32) flags |= 2;
+        conv32 = false;
+      }
       r2 = r2new;
       res = std::sqrt(r2) / bnorm;
       if (!hist.empty()) hist.back() = res;
@@ -1251,6 +1296,7 @@ int bdda_solve(bdda_plan* p, int solver, int precision, const void* B_dev, const
     }
     it = total > 0 ? total - 1 : 0;
     err = res;
+    flags |= std::min(cycles, 63) << 2;
   }
   if (true_res) {
     if (precision == BDDA_PREC_MIXED) {

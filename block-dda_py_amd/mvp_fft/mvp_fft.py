@@ -8,6 +8,7 @@ when every doubled axis length 2N is 5-smooth (the device then uses exactly
 the reference grid).  All arithmetic runs in libbdda's HIP kernels.
 """
 import os
+from collections import OrderedDict
 
 import numpy as np
 import torch
@@ -23,18 +24,44 @@ class GreenTensor:
     """Device-resident FFT of the interaction tensor (what ``fft_init`` returns).
 
     Holds the libbdda plan: complex128 and complex64 half spectra of the 6
-    unique components on the device.  ``shape`` is the convolution grid
-    (m0, m1, m2, 3, 3); m = 2N when 2N is 5-smooth, else the smallest
-    5-smooth length >= 2N - 1 (an exact linear convolution either way).
+    unique components on the device, shared with every other handle of the
+    same cuboid, lattice spacing and wavenumber while it is cached.
+    ``shape`` is the convolution grid (m0, m1, m2, 3, 3); m = 2N when 2N is
+    5-smooth, else the smallest 5-smooth length >= 2N - 1 (an exact linear
+    convolution either way).
     """
 
     dtype = np.dtype(np.complex128)
+
+    # Spectra built so far, by (cuboid, lattice_lf, k): a few spectrum-only
+    # plans kept alive so that the next instance with the same lattice and
+    # wavenumber (another m_p, another set of orientations) shares the
+    # spectrum instead of rebuilding it.  A_off does not depend on m_p.
+    cache_size = 4
+    _cache = OrderedDict()
+    builds = 0
 
     def __init__(self, n, lf, k, plan=None):
         self.n = tuple(int(v) for v in n)
         self.lf, self.k = float(lf), float(k)
         self.plan = plan if plan is not None else Plan(self.n)
-        self.plan.green_init(self.lf, self.k)
+        key = (self.n, self.lf, self.k)
+        holder = GreenTensor._cache.get(key)
+        if holder is None:
+            holder = Plan(self.n)
+            holder.green_init(self.lf, self.k)
+            GreenTensor.builds += 1
+            if GreenTensor.cache_size > 0:
+                GreenTensor._cache[key] = holder
+                while len(GreenTensor._cache) > GreenTensor.cache_size:
+                    GreenTensor._cache.popitem(last=False)
+        else:
+            GreenTensor._cache.move_to_end(key)
+        self.plan.share_green(holder)
+
+    @classmethod
+    def clear_cache(cls):
+        cls._cache.clear()
 
     @property
     def shape(self):

@@ -69,6 +69,16 @@ int bdda_plan_column_tile(const bdda_plan* plan, int L);
  * complex128 and complex64 (6 unique components, half spectrum).
  * Replaces build_interaction_tensor + fft_init (mvp_fft/mvp_fft.py:13-119). */
 int bdda_green_init(bdda_plan* plan, double lattice_lf, double k);
+/* Share src's Green spectrum with dst (same cuboid): dst then computes with
+ * it without building its own, and the spectrum lives until the last plan
+ * holding it is destroyed.  A later bdda_green_init on either plan builds a
+ * new spectrum for that plan only.  Lets one spectrum serve every
+ * DiscreteDipoles of a (lattice, lattice_lf, k) -- the reference rebuilds
+ * it per instance in set_interaction_matrix (bl_dda/scatterer.py:137-151),
+ * although A_off does not depend on m_p; its sweep (run_dda.py:92-99)
+ * varies m_p innermost. */
+int bdda_green_share(bdda_plan* dst, const bdda_plan* src);
+
 /* The full (m0, m1, m2, 3, 3) complex128 spectrum, i.e. what fft_init
  * returns when m = 2n on every axis (mvp_fft/mvp_fft.py:119). */
 int bdda_green_download(const bdda_plan* plan, void* out_host);
@@ -108,12 +118,16 @@ int bdda_block_mvp(bdda_plan* plan, const void* S_dev, void* AS_dev, int L, void
  *                 BDDA_PREC_MIXED: the operator's Goodman convolution in
  *                 complex64 (complex128 scatter / gather boundaries and
  *                 Jacobi epilogue); Krylov blocks, updates, block QR and
- *                 L x L solves in float64.  Every refine_every iterations
- *                 (0: never) the complex64 phase continues only while the
- *                 residual is on a log-linear course to tol by the next
- *                 check and the complex128 true residual is within 2x of
- *                 the recursive one; otherwise the solve restarts from its
- *                 float64 true residual with the complex128 convolution.
+ *                 L x L solves in float64.  Iterative refinement: each
+ *                 cycle solves for a correction on the float64 true
+ *                 residual.  Every refine_every iterations (0: never) a
+ *                 complex64 cycle continues only while its residual is on
+ *                 a log-linear course to tol by the next check and the
+ *                 complex128 true residual is within 2x of the recursive
+ *                 one.  A complex64 cycle that reaches its tolerance and at
+ *                 least halves the true residual is followed, if needed, by
+ *                 another complex64 cycle; otherwise the rest of the solve
+ *                 runs complex128 cycles.
  *   B_dev         (3N, L) complex128 right-hand side
  *   X0_dev        (3N, L) complex128 initial guess or NULL (zero start); a
  *                 guess whose preconditioned residual exceeds ||jpre B|| is
@@ -124,7 +138,11 @@ int bdda_block_mvp(bdda_plan* plan, const void* S_dev, void* AS_dev, int L, void
  *                 Frobenius over the block, / ||jpre B||
  *   true_res      recomputed ||jpre (B - A X)|| / ||jpre B|| (may be NULL)
  *   err_hist      per-iteration err (host, itermax doubles, may be NULL)
- *   flags_out     bit 0: the initial guess was used (may be NULL)
+ *   flags_out     bit 0: the initial guess was used; bit 1 (mixed): the
+ *                 solve fell back to complex128 cycles; bits 2-7 (mixed):
+ *                 refinement cycles run (at most 63); bits 8 and up: the
+ *                 number of err_hist entries written (may be NULL)
+ * 1 <= L <= 128.
  * Non-convergence is not an error: err_fin >= tol, as in the reference.
  * Replaces bl_cocg_rq_jacobi_mvp_fft and bl_bicgstab_jacobi_mvp_fft
  * (bl_krylov/bl_krylov.py:58-187). */

@@ -8,6 +8,7 @@
   depend on size: the mixed-precision solve reaches the float64 true
   residual and its S(0) / S_bk agree with the float64 solve within 1e-4.
 """
+import os
 import sys
 
 import numpy as np
@@ -67,3 +68,165 @@ def test_large_configuration_mixed_meets_fp64():
     sb = dd.compute_OCBS_observable_S_bk()
     for a, b in [(st, st64), (sp, sp64), (sb, sb64)]:
         assert np.max(np.abs(a - b) / np.abs(b)) < 1e-4
+
+
+# ------------------------------------------------ recorded reference particles
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _npz(name):
+    with np.load(os.path.join(GOLDEN, name)) as d:
+        return {k: d[k] for k in d.files}
+
+
+def test_voxeliser_on_reference_fields():
+    """Every workload particle built from its recorded coarse field on the
+    GPU voxeliser: lattice and mask equal the reference's (written by the
+    reference's gaussian_ellipsoid.py from the same field) bit for bit."""
+    from bdda_amd import workloads
+    fx = _npz("gre_fields.npz")
+    for w in workloads.all_workloads():
+        say(f"voxeliser {w['name']}")
+        name, n, lf, grid, is_in = workloads.build_geometry(w)
+        assert np.array_equal(n, fx[w["name"] + "__lattice_n"])
+        ref = np.unpackbits(fx[w["name"] + "__mask_bits"])[:is_in.size].astype(bool)
+        assert np.array_equal(is_in, ref), w["name"]
+
+
+def test_full64_fp64_matches_oracle():
+    """The benchmark particle (64^3 cuboid, birefringent, block-BiCGSTAB to
+    1e-5) for its first 8 orientations against the float64 oracle's solve on
+    the same particle (tests/golden/config_full64_L8.npz): same iteration
+    count and residual history, S(0), S_bk, C_abs, C_ext within 1e-6."""
+    from bdda_amd import workloads
+    from bl_dda.scatterer import DiscreteDipoles
+    DiscreteDipoles.verbose_default = False
+    ref = _npz("config_full64_L8.npz")
+    wl = workloads.FULL_64
+    say(f"{wl['name']}, first 8 orientations, fp64")
+    dd = workloads.make_dipoles(wl, precision="fp64", euler=ref["euler"])
+    assert dd._plan.n == (64, 64, 64)
+    assert np.array_equal(dd.lattice_n, ref["lattice_n"]) and dd.num_element_occupy == int(ref["N"])
+    assert dd.wl_0 == float(ref["wl_0"])
+    dd.solve_matrix_equation()
+    assert dd.converge and dd.iter_fin == int(ref["iter_fin"])
+    np.testing.assert_allclose(dd.err_history, ref["history"], rtol=1e-6)
+    st, sp = dd.compute_PCAS_observable_S_fw()
+    sb = dd.compute_OCBS_observable_S_bk()
+    for a, b in [(st, ref["S_fw_theta"]), (sp, ref["S_fw_phi"]), (sb, ref["S_bk"])]:
+        assert np.max(np.abs(a - b) / np.abs(b)) < 1e-6
+    np.testing.assert_allclose(dd.compute_C_abs(), ref["C_abs"], rtol=1e-6)
+    np.testing.assert_allclose(dd.compute_C_ext(), ref["C_ext"], rtol=1e-6)
+
+
+def test_full64_mixed_meets_fp64():
+    """The benchmark solve itself (L = 64): mixed precision reaches a true
+    residual below 1e-5 and its S(0), S_bk agree with the float64 solve
+    within 1e-4 of each quantity's largest magnitude over the orientations
+    (and within 1e-3 orientation by orientation); the float64 solve's true
+    residual is below 1e-5 too.  Both solves stop at a residual of 1e-5,
+    which bounds the amplitude errors relative to their overall size: an
+    orientation whose S_bk is 20x smaller than the typical one carries a 20x
+    larger relative error in either solve."""
+    from bdda_amd import workloads
+    from bl_dda.scatterer import DiscreteDipoles
+    DiscreteDipoles.verbose_default = False
+    wl = workloads.FULL_64
+    say(f"{wl['name']}, L = 64, fp64 then mixed")
+    dd = workloads.make_dipoles(wl, precision="fp64")
+    dd.solve_matrix_equation()
+    assert dd.converge and dd.true_residual < 1e-5
+    st64, sp64 = dd.compute_PCAS_observable_S_fw()
+    sb64 = dd.compute_OCBS_observable_S_bk()
+    dd.precision = "mixed"
+    dd.solve_matrix_equation()
+    assert dd.converge and dd.true_residual < 1e-5
+    st, sp = dd.compute_PCAS_observable_S_fw()
+    sb = dd.compute_OCBS_observable_S_bk()
+    for a, b in [(st, st64), (sp, sp64), (sb, sb64)]:
+        say(f"  max |dS| / max |S| = {np.max(np.abs(a - b)) / np.max(np.abs(b)):.2e}, "
+            f"max |dS / S| = {np.max(np.abs(a - b) / np.abs(b)):.2e}")
+        assert np.max(np.abs(a - b)) / np.max(np.abs(b)) < 1e-4
+        assert np.max(np.abs(a - b) / np.abs(b)) < 1e-3
+
+
+@pytest.fixture(scope="module")
+def batch32():
+    from bdda_amd import workloads
+    insts = workloads.batch_instances(workloads.BATCH_32)
+    say("batch of 32: geometry")
+    geos = [workloads.build_geometry(w) for w in insts]
+    # the oracle's float64 solves of the three smallest particles
+    from bl_dda.scatterer import DiscreteDipoles
+    DiscreteDipoles.verbose_default = False
+    small = sorted(range(len(insts)), key=lambda i: int(geos[i][4].sum()))[:3]
+    refs = {}
+    for i in small:
+        w = insts[i]
+        name, n, lf, grid, is_in = geos[i]
+        tg = osc.Target(n, lf, grid, is_in, np.array(w["m_p_xyz"], dtype=np.complex128))
+        wl0 = workloads.wavelength(w, lf, tg.N)
+        inc = osc.Incident(wl0, w["m_m"], workloads.euler_angles(w["L"], w["orient_seed"]))
+        say(f"oracle: instance {i}, N = {tg.N}")
+        refs[i] = osc.solve(tg, inc, "cocg", w["tol"], w["itermax"])
+    return insts, geos, refs
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch32_full(batch32, precision):
+    """All 32 uneven instances (lattices 15-84 cells, L = 16, block-COCG-rq
+    to 1e-5): each converges with a true residual below 1e-5; the batched
+    solve returns each instance's one-at-a-time result bit for bit; the three
+    smallest instances match the float64 oracle (iterations and S within 1e-6
+    in float64, S within 1e-4 in mixed precision)."""
+    from bdda_amd import workloads
+    from bdda_amd.batch import solve_batch
+    insts, geos, refs = batch32
+    say(f"batch of 32, {precision}")
+    dds = [workloads.make_dipoles(w, geometry=g, precision=precision) for w, g in zip(insts, geos)]
+    seq = []
+    for dd in dds:
+        dd.solve_matrix_equation()
+        assert dd.converge and dd.true_residual < 1e-5, dd.shape_name
+        seq.append((dd.X_device.clone(), dd.iter_fin, list(dd.err_history)))
+    solve_batch(dds, nstreams=8)
+    for dd, (X, it, hist) in zip(dds, seq):
+        assert torch.equal(dd.X_device, X) and dd.iter_fin == it and dd.err_history == hist
+    for i, ref in refs.items():
+        dd = dds[i]
+        tol = 1e-6 if precision == "fp64" else 1e-4
+        if precision == "fp64":
+            assert dd.iter_fin == ref["iter_fin"]
+        st, sp = dd.compute_PCAS_observable_S_fw()
+        sb = dd.compute_OCBS_observable_S_bk()
+        for a, b in [(st, ref["S_fw_theta"]), (sp, ref["S_fw_phi"]), (sb, ref["S_bk"])]:
+            assert np.max(np.abs(a - b) / np.abs(b)) < tol
+
+
+def test_warm_start_on_the_64_cube():
+    """The learned warm start at 64^3 (the 32^3-trained network run on a 64^3
+    cube) on the benchmark particle with mixed-precision BiCGSTAB: the
+    projected guess is either taken or dropped by the solver's rule, and the
+    solve converges to a true residual below 1e-5 either way (a dropped guess
+    gives exactly the zero-start iteration count)."""
+    from bdda_amd import workloads
+    from bdda_amd.fno import FNO
+    from bdda_amd.fno_train import warm_start
+    from bl_dda.scatterer import DiscreteDipoles
+    from conftest import PKG
+    DiscreteDipoles.verbose_default = False
+    say("warm start at 64^3")
+    model = FNO.load(os.path.join(PKG, "bdda_amd", "data", "fno_gre32.npz")).at_grid(64)
+    dd = workloads.make_dipoles(workloads.FULL_64, precision="mixed")
+    dd.solve_matrix_equation()
+    it0 = dd.iter_fin
+    assert dd.converge and dd.true_residual < 1e-5
+    res = []
+    dd.X0 = warm_start(model, dd, residuals=res)
+    dd.solve_matrix_equation()
+    say(f"zero start {it0 + 1} iterations; guess residual {res}; warm {dd.iter_fin + 1} iterations, "
+        f"used {dd.warm_start_used}")
+    assert dd.converge and dd.true_residual < 1e-5
+    assert res[1] <= res[0] + 1e-12 and res[1] <= 1.0 + 1e-12
+    if not dd.warm_start_used:
+        assert dd.iter_fin == it0

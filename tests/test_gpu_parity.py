@@ -386,6 +386,20 @@ def test_notebook_run_on_gpu(golden_nb):
     assert dd.converge and dd.iter_fin == int(nb["iter_fin"])
     assert ["%.4f" % e for e in dd.err_history] == ["%.4f" % e for e in nb["err_printed"]]
     np.testing.assert_allclose(dd.compute_OCBS_observable_S_bk(), nb["s_bk"], rtol=2e-6, atol=1e-8)
+    # cells 9-12: the volume-equivalent sphere's Mie values through the
+    # drop-in Mie module; the recorded per-orientation errors of Q_abs, Q_ext
+    # against Mie, and the notebook's 1% tolerance on S(0)
+    from analytical_scattering_theories.homogeneous_sphere import mie_compute_q_and_s
+    m_p_avg = (1.5 + 0.01j) * 3 / 3
+    q = mie_compute_q_and_s(float(nb["wl_0"]), float(nb["m_m"]), dd.ve_radius, m_p_avg, nang=3)
+    np.testing.assert_allclose(q[:3], nb["mie_q"], rtol=1e-13)
+    qabs = dd.compute_C_abs() / (np.pi * dd.ve_radius ** 2)
+    qext = dd.compute_C_ext() / (np.pi * dd.ve_radius ** 2)
+    np.testing.assert_allclose(np.abs(qabs / q[1] - 1), nb["qabs_relerr"], rtol=5e-5, atol=1e-8)
+    np.testing.assert_allclose(np.abs(qext / q[2] - 1), nb["qext_relerr"], rtol=5e-5, atol=1e-8)
+    st, sp = dd.compute_PCAS_observable_S_fw()
+    for s in (st, sp):
+        assert np.all(np.abs(s.real / q[3].real - 1) < 0.01) and np.all(np.abs(s.imag / q[3].imag - 1) < 0.01)
 
 
 def test_mixed_precision_meets_fp64_oracle():
