@@ -46,6 +46,7 @@ _SIGNATURES = {
     "bdda_profile_read": ([_vp, _pd, ctypes.POINTER(ctypes.c_int64)], _i),
     "bdda_rhs": ([_vp, _vp, _vp, _d, _i, _vp, _vp], _i),
     "bdda_observables": ([_vp, _vp, _i, _vp, _vp, _d, _vp, _vp], _i),
+    "bdda_observables_noalloc": ([_vp, _vp, _i, _vp, _vp, _d, _vp, _vp], _i),
     "bdda_nearest_distance": ([_vp, _i64, _vp, _i64, _vp], _i),
     "bdda_inside_mask": ([_vp, _i, _i, _i, _d, _vp], _i),
     "bdda_gemm_bf16": ([_i, _i, _i, _i, ctypes.c_float, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64,

@@ -162,6 +162,7 @@ void launch_rhs(double2* B, const double* pos, const double* u, const double2* e
 void launch_observables(const double2* X, const double* pos, const double* geo, const double2* epsfac,
                         double k, long long N, int L, double2* partial, int nparts, hipStream_t s);
 int obs_parts();
+void launch_obs_reduce(const double2* partial, int nparts, int L, double2* out, hipStream_t s);
 void launch_voxel_dist(const double* grid, long long n, const double* surf, long long P, double* dist,
                        hipStream_t s);
 void launch_voxel_inside(const double* dist, double thresh, int nx, int ny, int nz, unsigned char* inside,

@@ -707,6 +707,25 @@ void launch_observables(const double2* X, const double* pos, const double* geo, 
   hipLaunchKernelGGL(k_observables, dim3(nparts), dim3(256), 0, s, X, pos, geo, epsfac, k, N, L, partial);
 }
 
+// out[l][t] = sum over the workgroup partials in workgroup order (the
+// order the host-side sum of bdda_observables uses, so results are equal).
+__global__ void __launch_bounds__(256) k_obs_reduce(const double2* __restrict__ partial, int nparts, int L,
+                                                    double2* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // i = l * 5 + t
+  if (i >= L * 5) return;
+  double sx = 0.0, sy = 0.0;
+  for (int w = 0; w < nparts; ++w) {
+    const double2 v = partial[(long long)w * L * 5 + i];
+    sx += v.x;
+    sy += v.y;
+  }
+  out[i] = make_double2(sx, sy);
+}
+
+void launch_obs_reduce(const double2* partial, int nparts, int L, double2* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_obs_reduce, dim3((L * 5 + 255) / 256), dim3(256), 0, s, partial, nparts, L, out);
+}
+
 // ---------------------------------------------------------------- voxeliser
 // dist[i] = sqrt(min_j ((dx*dx + dy*dy) + dz*dz)), evaluated without
 // contraction so that every squared distance is the KD-tree's bit for bit

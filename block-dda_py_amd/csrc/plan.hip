@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <type_traits>
@@ -819,6 +820,44 @@ static int frob2(Ctx& c, const double2* A, double* out) { return dots_to_host<do
 
 using namespace bdda;
 
+// ------------------------------------------- allocation-free observables
+// Workgroup partial sums of bdda_observables_noalloc live in this
+// file-scope device array (one copy per GPU, held until the process ends)
+// instead of plan memory allocated on demand.  Host code never uses the
+// array's name as an address: each device's copy is looked up once with
+// hipGetSymbolAddress and the pointer kept per device.
+namespace {
+constexpr int kNoAllocMaxL = 256;
+constexpr size_t kObsScratchElems = 256 * (size_t)kNoAllocMaxL * 5;  // obs_parts() x L x 5
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {nullptr};
+}  // namespace
+static __device__ __align__(256) double2 scratch[kObsScratchElems];
+
+// The calling thread's current device's copy of `scratch` (looked up on
+// the first call on that device).
+static int scratch_ptr(void** out) {
+  int dev = 0;
+  BDDA_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("allocation-free observables: device index out of range");
+    return BDDA_E_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return BDDA_E_HIP;
+    }
+    g_scratch_dev[dev] = ptr;
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
 // ====================================================================== C ABI
 extern "C" {
 
@@ -1485,6 +1524,37 @@ int bdda_observables(bdda_plan* p, const void* X_dev, int L, const double* geo, 
       }
       out[l * 5 + t] = make_double2(sx, sy);
     }
+  return 0;
+}
+
+int bdda_observables_noalloc(bdda_plan* p, const void* X_dev, int L, const double* geo_dev, const void* epsfac_dev,
+                             double k, void* out_dev, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  g_err.clear();
+  if (!p->target_ready || !p->has_pos || !X_dev || !geo_dev || !epsfac_dev || !out_dev) {
+    set_error("bdda_observables_noalloc: bad arguments or no target");
+    return BDDA_E_ARG;
+  }
+  if (L < 1 || L > kNoAllocMaxL) {
+    set_error("bdda_observables_noalloc: L = " + std::to_string(L) + " is outside 1.." +
+              std::to_string(kNoAllocMaxL) + " (the size of its fixed scratch buffer)");
+    return BDDA_E_ARG;
+  }
+  const int np = obs_parts();
+  if ((size_t)np * L * 5 > kObsScratchElems) {
+    set_error("bdda_observables_noalloc: partial sums exceed the scratch buffer");
+    return BDDA_E_ARG;
+  }
+  void* part = nullptr;
+  if (int rc = scratch_ptr(&part)) return rc;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  // every partial slot (workgroup < np, l < L) is written by the kernel
+  // before the reduction reads it, so the scratch needs no clearing
+  launch_observables((const double2*)X_dev, p->pos.as<double>(), geo_dev, (const double2*)epsfac_dev, k, p->N, L,
+                     (double2*)part, np, s);
+  BDDA_HIP(hipGetLastError());
+  launch_obs_reduce((const double2*)part, np, L, (double2*)out_dev, s);
+  BDDA_HIP(hipGetLastError());
   return 0;
 }
 

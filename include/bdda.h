@@ -204,6 +204,20 @@ int bdda_rhs(bdda_plan* plan, const double* u_host, const void* e0_host, double 
 int bdda_observables(bdda_plan* plan, const void* X_dev, int L, const double* geo_host, const void* epsfac_host,
                      double k, void* out_host, void* stream);
 
+/* bdda_observables without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  Inputs and output are device pointers: geo_dev (L, 24)
+ * float64 and epsfac_dev (3) complex128 as for bdda_observables, out_dev
+ * (L, 5) complex128 receives the same sums (equal to bdda_observables').
+ * Asynchronous on `stream`; calls no hipMalloc / hipFree / hipMemcpy.
+ * The workgroup partials go to a fixed per-GPU scratch buffer of
+ * 256 x 256 x 5 complex128 (5.2 MB), so 1 <= L <= 256 (else BDDA_E_ARG),
+ * and calls that overlap on one GPU (other streams, other threads) race
+ * on that buffer: serialise them. */
+int bdda_observables_noalloc(bdda_plan* plan, const void* X_dev, int L, const double* geo_dev, const void* epsfac_dev,
+                             double k, void* out_dev, void* stream);
+
 /* Nearest surface-point distance of every lattice point, bit-exact with
  * the KD-tree query: sqrt(min_j ((dx*dx + dy*dy) + dz*dz)).
  * grid_host (n, 3), surf_host (P, 3) float64; dist_host (n) float64.
