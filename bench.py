@@ -2,15 +2,16 @@
 """Benchmark of the block-DDA solve on one MI355X.
 
 Workload (one "step" = one full solve): GRE particle bc = 0.8, ab = 0.7,
-beta = 0.2 (seed 0) on a 54 x 51 x 63 lattice (21,434 dipoles) padded to a
-64^3 cuboid (128^3 doubled FFT grid), birefringent m_p = (1.55, 1.60, 1.50)
-+ 0.001i, m_m = 1.33, L = 64 orientations, complex128 block-BiCGSTAB to
-relative residual 1e-5 with B resident on the device.  A float64 solve stops
-on its recursive residual, as the reference's solver does; its true
-residual is computed after the timed solves and reported.  A mixed solve
-checks its true residual as part of the method.  Prints one JSON line.
+beta = 0.2 (seed 0, its recorded coarse field) on a 51 x 48 x 63 lattice
+(21,113 dipoles) padded to a 64^3 cuboid (128^3 doubled FFT grid),
+birefringent m_p = (1.55, 1.60, 1.50) + 0.001i, m_m = 1.33, L = 64
+orientations, complex128 block-BiCGSTAB to relative residual 1e-5 with B
+resident on the device.  Every timed float64 solve recomputes its true
+residual (one more operator application) after stopping on its recursive
+one, as the reference's solver stops; a mixed solve checks its true
+residual as part of the method.  Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-mixed] [--no-large] [--no-batch]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-mixed] [--no-warm] [--no-large] [--no-batch]
                     [--no-cpu-baseline]
 """
 import argparse
@@ -95,8 +96,10 @@ def pass_bytes(dims, m, N, L, Lb, s, extent=None, lines=None):
 def timed_solves(dd, precision, steps, warmup, torch, profile_steps=1):
     """Warm-up, then `steps` timed solves with nothing else recorded; then
     `profile_steps` further solves with per-pass HIP events on the solve's
-    stream for the pass timings (events are kept out of the timed solves)."""
+    stream for the pass timings (events are kept out of the timed solves).
+    Every solve ends with its true residual computed."""
     dd.precision = precision
+    dd.check_true_residual = True
     for _ in range(warmup):
         dd.solve_matrix_equation()
     torch.cuda.synchronize()
@@ -114,33 +117,110 @@ def timed_solves(dd, precision, steps, warmup, torch, profile_steps=1):
     return dt, prof
 
 
-def cpu_baseline(geometry, wl, gpu_iters, sample_L=16):
-    """The float64 NumPy/SciPy oracle (the reference algorithm) on the host:
-    one block-BiCGSTAB iteration at sample_L columns on the particle's own
-    (unpadded) lattice, scaled to L columns and the GPU iteration count."""
+_CPU_SAMPLE = r"""
+import json, os, sys, time
+import numpy as np
+sys.path.insert(0, sys.argv[2])
+from oracle import bl_krylov as okr, scatterer as osc
+from oracle.mvp_fft import fft_init
+d = np.load(sys.argv[1])
+tg = osc.Target(d["n"], float(d["lf"]), d["grid"], d["is_in"], d["m_p"])
+inc = osc.Incident(float(d["wl0"]), float(d["m_m"]), d["euler"])
+eps_r, alpha, diag = osc.polarizability(tg, inc)
+At = fft_init(tg.lattice_n, 3, tg.lf, inc.k)
+E_inc, B = osc.rhs(tg, inc)
+iters = int(sys.argv[3])
+t0 = time.perf_counter()
+okr.bl_bicgstab(tg.lattice_n, 3, tg.address, At, diag, B, 0.0, iters)
+print(json.dumps({"seconds": time.perf_counter() - t0}))
+"""
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(geometry, dd, wl, gpu_iters, max_iters=6):
+    """The float64 NumPy/SciPy oracle (the reference's algorithm: scipy.fft,
+    numpy matmul and LAPACK solves in its order) on the host cores: a child
+    process with the FFT workers and BLAS threads set to the cores it may
+    use (the box's CPU share: 16), timing min(max_iters, GPU iterations)
+    block-BiCGSTAB iterations at the full L on the particle's own
+    (unpadded) lattice -- the whole solve when the GPU needs no more --
+    scaled to the GPU's float64 iteration count."""
+    sample_iters = min(max_iters, gpu_iters)
+    import subprocess
+    import tempfile
     import numpy as np
     cores = min(16, len(os.sched_getaffinity(0)))
-    os.environ["DDA_FFT_WORKERS"] = str(cores)
-    from oracle import bl_krylov as okr
-    from oracle import scatterer as osc
-    from bdda_amd.workloads import euler_angles, wavelength
     name, n, lf, grid, is_in = geometry
-    m_p = np.array(wl["m_p_xyz"], dtype=np.complex128)
-    tg = osc.Target(n, lf, grid, is_in, m_p)
-    inc = osc.Incident(wavelength(wl, lf, tg.N), wl["m_m"], euler_angles(wl["L"], wl["orient_seed"])[:sample_L])
-    eps_r, alpha, diag = osc.polarizability(tg, inc)
-    from oracle.mvp_fft import fft_init
-    At = fft_init(tg.lattice_n, 3, tg.lf, inc.k)
-    E_inc, B = osc.rhs(tg, inc)
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "sample.npz")
+        np.savez(path, n=np.asarray(n), lf=lf, grid=grid, is_in=is_in,
+                 m_p=np.array(wl["m_p_xyz"], dtype=np.complex128), wl0=dd.wl_0, m_m=wl["m_m"],
+                 euler=np.asarray(dd.euler_angles))
+        env = dict(os.environ, DDA_FFT_WORKERS=str(cores))
+        for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
+            env.setdefault(k, str(cores))
+        r = subprocess.run([sys.executable, "-c", _CPU_SAMPLE, path, ROOT, str(sample_iters)], env=env,
+                           capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError("cpu baseline failed: " + r.stderr[-2000:])
+    t = json.loads(r.stdout.strip().splitlines()[-1])["seconds"]
+    per_iter = t / sample_iters
+    total = per_iter * gpu_iters
+    return {"value": round(wl["L"] / total, 4), "unit": "orientations/s", "cores": cores, "kind": "port",
+            "cpu_model": cpu_model(), "threads": {"DDA_FFT_WORKERS": cores,
+                                                  "OMP_NUM_THREADS": env["OMP_NUM_THREADS"],
+                                                  "OPENBLAS_NUM_THREADS": env["OPENBLAS_NUM_THREADS"]},
+            "sample": f"{sample_iters} block-BiCGSTAB iterations of the float64 NumPy/SciPy oracle at L={wl['L']} "
+                      f"on the particle's own {tuple(int(v) for v in n)} lattice ({t:.1f} s)"
+                      + ("" if sample_iters == gpu_iters else
+                         f", extrapolated to {gpu_iters} iterations (the GPU float64 count; same algorithm)")
+                      + ", without a true-residual check"}
+
+
+def warm_leg(dd, steps, torch, t_fp64, S_fw64, S_bk64):
+    """Mixed precision with the learned warm start (the network trained on
+    32^3 particles, run on the 64^3 cube): each step runs the network's
+    forward pass for all L orientations, the guess's minimal-residual
+    projection (one float64 operator application) and the mixed solve, whose
+    rule keeps the guess only if its residual is below that of zero."""
+    import numpy as np
+    from bdda_amd.fno import FNO
+    from bdda_amd.fno_train import warm_start
+    model = FNO.load(os.path.join(ROOT, "block-dda_py_amd", "bdda_amd", "data", "fno_gre32.npz")).at_grid(64)
+    dd.precision = "mixed"
+    res = []
+
+    def step():
+        dd.X0 = warm_start(model, dd, residuals=res)
+        dd.solve_matrix_equation()
+
+    step()
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    okr.bl_bicgstab(tg.lattice_n, 3, tg.address, At, diag, B, 0.0, 1)
-    t_iter = time.perf_counter() - t0
-    per_iter_full = t_iter * wl["L"] / sample_L
-    total = per_iter_full * gpu_iters
-    return {"value": wl["L"] / total, "unit": "orientations/s", "cores": cores, "kind": "port",
-            "sample": f"1 block-BiCGSTAB iteration (float64 NumPy/SciPy oracle, {cores} threads) at L={sample_L} "
-                      f"on the particle's own {tuple(int(v) for v in n)} lattice ({t_iter:.2f} s), scaled x"
-                      f"{wl['L'] // sample_L} in L and x{gpu_iters} iterations (the GPU float64 count)"}
+    for _ in range(steps):
+        res.clear()
+        step()
+    torch.cuda.synchronize()
+    dtw = (time.perf_counter() - t0) / steps
+    S_fw = np.stack(dd.compute_PCAS_observable_S_fw())
+    S_bk = dd.compute_OCBS_observable_S_bk()
+    out = {"value": round(dd.L / dtw, 3), "ms_per_step": round(dtw * 1e3, 2), "iterations": dd.iter_fin + 1,
+           "guess_used": bool(dd.warm_start_used), "guess_residual_raw": res[0], "guess_residual_projected": res[1],
+           "true_residual": dd.true_residual, "speedup_vs_fp64": round(t_fp64 / dtw, 3),
+           "S_fw_err_vs_fp64_of_max": float(np.max(np.abs(S_fw - S_fw64)) / np.max(np.abs(S_fw64))),
+           "S_bk_err_vs_fp64_of_max": float(np.max(np.abs(S_bk - S_bk64)) / np.max(np.abs(S_bk64))),
+           "network": "fno_gre32 (4 Fourier layers, width 32, 12 modes) at grid 64"}
+    dd.X0 = None
+    return out
 
 
 def large_leg(torch, steps=2):
@@ -160,6 +240,8 @@ def large_leg(torch, steps=2):
         S[prec] = (np.stack(dd.compute_PCAS_observable_S_fw()), dd.compute_OCBS_observable_S_bk())
         out[prec] = {"value": round(wl["L"] * steps / dt, 2), "ms_per_solve": round(dt / steps * 1e3, 2),
                      "iterations": dd.iter_fin + 1, "true_residual": dd.true_residual}
+        if prec == "mixed":
+            out[prec]["refinement"] = dd.solve_info
     out["mixed"]["speedup_vs_fp64"] = round(out["mixed"]["value"] / out["fp64"]["value"], 3)
     out["mixed"]["S_fw_max_rel_err_vs_fp64"] = float(np.max(np.abs(S["mixed"][0] - S["fp64"][0]) / np.abs(S["fp64"][0])))
     out["mixed"]["S_bk_max_rel_err_vs_fp64"] = float(np.max(np.abs(S["mixed"][1] - S["fp64"][1]) / np.abs(S["fp64"][1])))
@@ -223,6 +305,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-warm", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-large", action="store_true")
@@ -254,17 +337,8 @@ def main():
     dt, prof = timed_solves(dd, "fp64", a.steps, a.warmup, torch)
     iters = dd.iter_fin + 1
     tr = dd.true_residual
-    # the float64 solve stops on its recursive residual, as the reference's
-    # does; what checking its true residual adds (one complex128 operator
-    # application and two norms), timed on its own
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        plan.true_residual(dd._B_dev, dd.X_device)
-    torch.cuda.synchronize()
-    dt_check = (time.perf_counter() - t0) / a.steps
-    log(f"fp64: {dt / a.steps * 1e3:.1f} ms/solve, {iters} iterations, true residual {tr:.2e} "
-        f"(checking it: {dt_check * 1e3:.2f} ms)")
+    log(f"fp64: {dt / a.steps * 1e3:.1f} ms/solve with its true-residual check, {iters} iterations, "
+        f"true residual {tr:.2e}")
 
     Lb = plan.column_tile(L)
     ext, nlines = dd._plan.occupied_extent, dd._plan.occupied_lines
@@ -301,10 +375,9 @@ def main():
                    "fft_grid": list(m), "dipoles": int(N), "L": L, "solver": wl["solver"], "tol": wl["tol"],
                    "precision": "fp64"},
         "iterations": iters, "true_residual": tr,
-        "true_residual_check": {"ms": round(dt_check * 1e3, 3),
-                                "value_including_it": round(L / (dt / a.steps + dt_check), 3),
-                                "note": "value stops on the recursive residual like the reference's solver; "
-                                        "this adds the true-residual check to every solve"},
+        "true_residual_check": "inside every timed solve: the float64 solve stops on its recursive residual "
+                               "like the reference's, then recomputes ||jpre(B - A X)|| / ||jpre B|| (one "
+                               "complex128 operator application)",
         "roofline": roofline,
         "mvp": {"ms": round(mvp_ms, 4), "alg_bytes": mvp_alg, "alg_GBs": round(mvp_alg / mvp_ms / 1e6, 1),
                 "alg_bytes_note": "16 (26 n 3L + 6 M) over the whole cuboid; the passes skip the lines and rows "
@@ -331,13 +404,21 @@ def main():
                 mp[nm] = {"avg_ms": round(ms / cnt, 4), "launches": cnt, "GBs": round(pbm[nm] / (ms / cnt) / 1e6, 1)}
         result["mixed"] = {
             "value": round(L * a.steps / dtm, 3), "ms_per_step": round(dtm / a.steps * 1e3, 2),
-            "iterations": dd.iter_fin + 1, "true_residual": dd.true_residual,
+            "iterations": dd.iter_fin + 1, "true_residual": dd.true_residual, "refinement": dd.solve_info,
             "speedup_vs_fp64": round(dt / dtm, 3),
             "S_fw_max_rel_err_vs_fp64": float(np.max(np.abs(S_fwm - S_fw64) / np.abs(S_fw64))),
             "S_bk_max_rel_err_vs_fp64": float(np.max(np.abs(S_bkm - S_bk64) / np.abs(S_bk64))),
+            "S_fw_err_vs_fp64_of_max": float(np.max(np.abs(S_fwm - S_fw64)) / np.max(np.abs(S_fw64))),
+            "S_bk_err_vs_fp64_of_max": float(np.max(np.abs(S_bkm - S_bk64)) / np.max(np.abs(S_bk64))),
             "passes_c64": mp,
         }
         log(f"mixed: {dtm / a.steps * 1e3:.1f} ms/solve")
+
+    if not a.no_mixed and not a.no_warm:
+        log("mixed + learned warm start")
+        result["mixed_warm"] = warm_leg(dd, a.steps, torch, dt / a.steps, S_fw64, S_bk64)
+        w = result["mixed_warm"]
+        log(f"mixed + warm: {w['ms_per_step']} ms/solve, {w['iterations']} iterations, guess used {w['guess_used']}")
 
     if not a.no_large:
         log("96^3 configuration")
@@ -354,7 +435,7 @@ def main():
 
     if not a.no_cpu_baseline:
         log("cpu baseline")
-        result["cpu_baseline"] = cpu_baseline(geometry, wl, iters)
+        result["cpu_baseline"] = cpu_baseline(geometry, dd, wl, iters)
     print(json.dumps(result), flush=True)
 
 

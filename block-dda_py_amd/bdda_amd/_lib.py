@@ -41,6 +41,7 @@ _SIGNATURES = {
     "bdda_block_mvp": ([_vp, _vp, _vp, _i, _vp], _i),
     "bdda_solve": ([_vp, _i, _i, _vp, _vp, _vp, _i, _d, _i, _i, _pi, _pd, _pd, _pd, _pi, _vp], _i),
     "bdda_project_guess": ([_vp, _vp, _vp, _i, _pd, _pd, _vp], _i),
+    "bdda_true_residual": ([_vp, _vp, _vp, _i, _pd, _vp], _i),
     "bdda_solve_batch": ([_i, _vp, _i, _i, _vp, _vp, _vp, _vp, _d, _i, _i, _i, _pi, _pd, _pd, _pd, _pi], _i),
     "bdda_profile_enable": ([_vp, _i], _i),
     "bdda_profile_read": ([_vp, _pd, ctypes.POINTER(ctypes.c_int64)], _i),

@@ -130,7 +130,6 @@ class Plan:
         # z lines (ix, iy) holding an occupied cell: the z passes run only these
         self.occupied_lines = int(np.unique(address // nz).size)
         self.target_key = key
-        self._jpre_dev = torch.as_tensor(jpre, device="cuda")
 
     def block_mvp(self, S):
         S = to_device(S)
@@ -170,10 +169,11 @@ class Plan:
     def true_residual(self, B, X):
         """||jpre (B - A X)||_F / ||jpre B||_F with one complex128 operator
         application (B, X: (3N, L) complex128 device tensors)."""
-        B = to_device(B)
-        Bj = B * self._jpre_dev[:, None]
-        R = Bj - self.block_mvp(X)
-        return float(torch.linalg.norm(R) / torch.linalg.norm(Bj))
+        B, X = to_device(B), to_device(X)
+        r = ctypes.c_double(0.0)
+        check(self.lib.bdda_true_residual(self.h, ptr(B), ptr(X), B.shape[1], ctypes.byref(r), _stream()),
+              "true_residual")
+        return r.value
 
     def project_guess(self, B, X0):
         """X0 (3N, L) complex128 device tensor, in place -> X0 C minimising

@@ -111,6 +111,10 @@ class DiscreteDipoles(Target, IncidentField):
         self.solver = "bicgstab"
         self.precision = "fp64"
         self.refine_every = 20
+        # a float64 solve stops on its recursive residual, as the reference's
+        # does; True recomputes the true residual within the solve (one more
+        # operator application) instead of on first access
+        self.check_true_residual = False
         self.X0 = None
         self.pad_to = None
         self.verbose = DiscreteDipoles.verbose_default
@@ -180,7 +184,7 @@ class DiscreteDipoles(Target, IncidentField):
             print("Starting {} iterative solver...".format(self._solver_name()))
         X, it, err, tr, hist, warm = self._plan.solve(self._B_dev, self.solver, self.precision, self.tol,
                                                       self.itermax, X0=self.X0, refine_every=self.refine_every,
-                                                      true_residual=False)
+                                                      true_residual=self.check_true_residual)
         self.solve_info = dict(self._plan.last_solve_info)
         self._store_solution(X, it, err, tr, hist, warm, time.time() - start)
 

@@ -1356,6 +1356,35 @@ int bdda_solve(bdda_plan* p, int solver, int precision, const void* B_dev, const
   return 0;
 }
 
+int bdda_true_residual(bdda_plan* p, const void* B_dev, const void* X_dev, int L, double* res, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  g_err.clear();
+  if (!p->green_ready || !p->target_ready) {
+    set_error("bdda_true_residual: call bdda_green_init and bdda_set_target first");
+    return BDDA_E_STATE;
+  }
+  if (L < 1 || !B_dev || !X_dev || !res) {
+    set_error("bdda_true_residual: bad arguments");
+    return BDDA_E_ARG;
+  }
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  Ctx c{p, L, 3 * p->N, s, nullptr};
+  const size_t b64 = (size_t)c.rows * L * sizeof(double2);
+  int rc = p->Bj64.ensure(b64);
+  if (!rc) rc = p->blk[8].ensure(b64);
+  if (rc) return rc;
+  double2* Bj = p->Bj64.as<double2>();
+  double2* tmp = p->blk[8].as<double2>();
+  launch_scale_rows<double2>(Bj, (const double2*)B_dev, p->jpre64.as<double2>(), c.rows, L, s);
+  BDDA_HIP(hipGetLastError());
+  double bn2 = 0.0, r2 = 0.0;
+  if ((rc = frob2(c, Bj, &bn2))) return rc;
+  if ((rc = apply_operator<double2>(p, (const double2*)X_dev, tmp, L, true, s))) return rc;
+  if ((rc = axpy_norm_host<double2>(c, tmp, Bj, zc(-1.0), tmp, &r2))) return rc;
+  *res = bn2 > 0.0 ? std::sqrt(r2 / bn2) : 0.0;
+  return 0;
+}
+
 int bdda_solve_batch(int nb, bdda_plan* const* plans, int solver, int precision, const void* const* B,
                      const void* const* X0, void* const* X, const int* L, double tol, int itermax, int refine_every,
                      int nstreams, int* iter_fin, double* err_fin, double* true_res, double* err_hist,

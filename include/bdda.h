@@ -44,9 +44,11 @@ const char* bdda_last_error(void);
 int bdda_version(void);
 
 /* FFT kernel selection for the whole process: 0 (default) uses the
- * two-step register kernels for lengths m = R1*R2 (R1, R2 in {2,3,4,5,8,16}),
- * with the dedicated x-pass kernel where the 8-column tile allows it, and
- * the runtime-radix LDS Stockham kernels otherwise; 1 forces the
+ * two-step register kernels for the 5-smooth lengths m = R1*R2 with R1, R2
+ * in {3,4,5,6,8,9,10,12,15,16} (32 lengths from 16 to 256), with the
+ * dedicated x-pass kernel (forward x FFT, Green multiply, inverse x FFT in
+ * one launch) where R1 is even on the compiled 8- and 4-column tiles, and
+ * the runtime-radix LDS Stockham kernels for other lengths; 1 forces the
  * runtime-radix kernels everywhere; 2 is 0 without the dedicated x-pass
  * kernel (the generic register kernel runs the x pass).  1 and 2 exist to
  * cross-check the kernels. */
@@ -149,6 +151,13 @@ int bdda_block_mvp(bdda_plan* plan, const void* S_dev, void* AS_dev, int L, void
 int bdda_solve(bdda_plan* plan, int solver, int precision, const void* B_dev, const void* X0_dev, void* X_dev,
                int L, double tol, int itermax, int refine_every, int* iter_fin, double* err_fin,
                double* true_res, double* err_hist, int* flags_out, void* stream);
+
+/* True preconditioned relative residual ||jpre (B - A X)||_F / ||jpre B||_F
+ * of a solution X_dev for B_dev, both (3N, L) complex128: one complex128
+ * operator application and two norms (the stopping metric of
+ * bl_krylov/bl_krylov.py:87-88,106,177, recomputed from X).  Host scalar
+ * out; synchronises the stream. */
+int bdda_true_residual(bdda_plan* plan, const void* B_dev, const void* X_dev, int L, double* res, void* stream);
 
 /* Minimal-residual projection of an initial guess: X0_dev (3N, L)
  * complex128 is replaced by X0 C, where the L x L matrix C minimises
