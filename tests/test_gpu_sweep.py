@@ -141,6 +141,16 @@ def test_sweep_on_the_reference_conditions(tmp_path):
             d = np.max(np.abs(got[idx] - want[idx]))
             assert (d == 0 if den == 0 else d / den <= max(1e-6, 10 * sens[idx])), (k, idx, d / den, sens[idx])
         assert rel(got, sd(k)) < 2e-2, k
+    # CAS-v2 post-processing of the GPU results classifies every orientation
+    # as the oracle's results do
+    from bdda_amd import cas
+    for ip in range(want.shape[0]):
+        for im in range(want.shape[1]):
+            got_f = [c["non_sphere_fraction"] for c in cas.cas_v2(out, ip, im)]
+            ref_f = [cas.non_sphere_fraction(cas.pcas_depolarization(
+                ref["S_fw_PCAS_theta"][ip, im, ir, 0, 0, 0], ref["S_fw_PCAS_phi"][ip, im, ir, 0, 0, 0])
+                ["Tot_depol_SNR2"]) for ir in range(want.shape[2])]
+            assert got_f == ref_f, (ip, im, got_f, ref_f)
     # resume: a second run finds every condition done
     again = sweep.open_store(path)
     s2 = sweep.run_sweep(again, log=lambda m: None)
