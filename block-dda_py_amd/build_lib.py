@@ -27,7 +27,7 @@ FLAGS = [
     "-I", os.path.join(ROOT, "include"),
 ]
 FFT_SOURCES = ["kernels_fft.hip", "fft_len_a.hip", "fft_len_b.hip", "fft_len_c.hip", "fft_len_d.hip"]
-SOURCES = FFT_SOURCES + ["kernels_block.hip", "kernels_fno.hip", "plan.hip"]
+SOURCES = FFT_SOURCES + ["kernels_block.hip", "kernels_dla.hip", "kernels_fno.hip", "plan.hip"]
 # the FFT kernels fold the additions of their known-zero (pruned) inputs,
 # which IEEE signed zeros would otherwise keep
 FILE_FLAGS = {src: ["-fno-signed-zeros"] for src in FFT_SOURCES}

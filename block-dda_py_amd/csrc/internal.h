@@ -113,11 +113,12 @@ int gram_parts();
 // Y = a*X + sum_j (W1 + w2c*W2)[i][j] C[j][l] + e*E ;  X, W2, E nullable.
 template <typename V>
 void launch_update(V* Y, const V* X, double2 a, const V* W1, const V* W2, double2 w2c,
-                   const double2* C, const V* E, double2 e, long long rows, int L, hipStream_t s);
+                   const double2* C, const V* E, double2 e, long long rows, int L, hipStream_t s,
+                   const double2* sc = nullptr);  // sc (device, nullable): w2c = sc[0], e = sc[1]
 // Y = X + c*Z, and partial |Y|^2 sums (nullable).
 template <typename V>
 void launch_axpy_norm(V* Y, const V* X, double2 c, const V* Z, long long n, double* partial,
-                      int nparts, hipStream_t s);
+                      int nparts, hipStream_t s, const double2* c_dev = nullptr);  // c = *c_dev if given
 // partial sums of tr(A^H B) and |B|^2 (each nullable) -> double2/double partials
 template <typename V>
 void launch_dot(const V* A, const V* B, long long n, double2* part_dot, double* part_nrm,
@@ -162,6 +163,25 @@ void launch_rhs(double2* B, const double* pos, const double* u, const double2* e
 void launch_observables(const double2* X, const double* pos, const double* geo, const double2* epsfac,
                         double k, long long N, int L, double2* partial, int nparts, hipStream_t s);
 int obs_parts();
+
+// --------------------------------------------- device L x L algebra (L <= 64)
+constexpr int kDlaMax = 64;
+enum { DLA_SINGULAR = 1, DLA_NOT_PD = 2, DLA_SHIFTED = 4 };
+// X = A^-1 op(B) (n x n A, n x m B, op: negation when neg_b); Xneg = -X,
+// LU / piv (for dla_lu_resolve) nullable; DLA_SINGULAR in *status, and zero
+// outputs, on an exactly singular A or when *status already holds it.
+void dla_lu_solve(const double2* A, const double2* B, int n, int m, bool neg_b, double2* X, double2* Xneg,
+                  double2* LU, int* piv, int* status, hipStream_t s);
+void dla_lu_resolve(const double2* LU, const int* piv, const double2* B, int n, int m, bool neg_b, double2* X,
+                    const int* status, hipStream_t s);
+// R^H R = G (+ shift I if the plain factorisation fails and shift is set);
+// R upper, Ri = R^-1.
+void dla_chol_inv(const double2* G, int n, double rows, bool shift, double2* R, double2* Ri, int* status,
+                  hipStream_t s);
+void dla_matmul(const double2* A, bool ta, const double2* B, double2* C, int n, hipStream_t s);
+void dla_frob(const double2* A, int n, double scale, double* out, hipStream_t s);
+void dla_xi(const double2* pdot, const double* pnrm, int np, const int* status, double2* sc, hipStream_t s);
+void dla_norm_err(const double* pnrm, int np, double scale, double* out, hipStream_t s);
 void launch_obs_reduce(const double2* partial, int nparts, int L, double2* out, hipStream_t s);
 void launch_voxel_dist(const double* grid, long long n, const double* surf, long long P, double* dist,
                        hipStream_t s);

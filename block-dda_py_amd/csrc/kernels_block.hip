@@ -285,8 +285,12 @@ template void launch_gram<float2>(const float2*, const float2*, long long, int, 
 template <typename V, int TL>
 __global__ void __launch_bounds__(256) k_update(V* Y, const V* X, double2 a, const V* W1, const V* W2,
                                                 double2 w2c, const double2* __restrict__ C, const V* E,
-                                                double2 e, long long rows, int L) {
+                                                double2 e, long long rows, int L, const double2* sc) {
   using T = typename RealOf<V>::T;
+  if (sc) {
+    w2c = sc[0];
+    e = sc[1];
+  }
   constexpr int TQ = TL / 4, RB = 4096 / TL, RQ = RB / 4, KC = TL == 16 ? 8 : 16;
   __shared__ double2 Ws[RB][KC + 1];
   __shared__ double2 Cs[KC][TL];
@@ -382,8 +386,12 @@ template <int KS>
 __global__ void __launch_bounds__(256) k_update_mfma(double2* Y, const double2* X, double2 a, const double2* W1,
                                                      const double2* W2, double2 w2c,
                                                      const double2* __restrict__ C, const double2* E, double2 e,
-                                                     long long rows, int L) {
+                                                     long long rows, int L, const double2* sc) {
   __shared__ double2 Cs[4 * KS][64];
+  if (sc) {
+    w2c = sc[0];
+    e = sc[1];
+  }
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int kr = lane >> 4, cc = lane & 15;
   for (int idx = tid; idx < 4 * KS * 64; idx += 256) {
@@ -449,35 +457,39 @@ __global__ void __launch_bounds__(256) k_update_mfma(double2* Y, const double2* 
 
 template <typename V, int TL>
 static void launch_update_tile(V* Y, const V* X, double2 a, const V* W1, const V* W2, double2 w2c,
-                               const double2* C, const V* E, double2 e, long long rows, int L, hipStream_t s) {
+                               const double2* C, const V* E, double2 e, long long rows, int L, hipStream_t s,
+                               const double2* sc) {
   constexpr int RB = 4096 / TL;
   if constexpr (std::is_same<V, double2>::value && TL == 64) {
     if (L <= 64) {  // one column tile: the matrix-core kernel
       const long long slabs = (rows + 63) / 64;
       const int g = (int)min(slabs, 1024LL);
-      hipLaunchKernelGGL((k_update_mfma<16>), dim3(g), dim3(256), 0, s, Y, X, a, W1, W2, w2c, C, E, e, rows, L);
+      hipLaunchKernelGGL((k_update_mfma<16>), dim3(g), dim3(256), 0, s, Y, X, a, W1, W2, w2c, C, E, e, rows, L,
+                         sc);
       return;
     }
   }
   const long long slabs = (rows + RB - 1) / RB;
   const int g = (int)min(slabs, 4096LL);
   dim3 grid(g, (L + TL - 1) / TL);
-  hipLaunchKernelGGL((k_update<V, TL>), grid, dim3(256), 0, s, Y, X, a, W1, W2, w2c, C, E, e, rows, L);
+  hipLaunchKernelGGL((k_update<V, TL>), grid, dim3(256), 0, s, Y, X, a, W1, W2, w2c, C, E, e, rows, L, sc);
 }
 
 template <typename V>
 void launch_update(V* Y, const V* X, double2 a, const V* W1, const V* W2, double2 w2c, const double2* C,
-                   const V* E, double2 e, long long rows, int L, hipStream_t s) {
+                   const V* E, double2 e, long long rows, int L, hipStream_t s, const double2* sc) {
   switch (block_tile(L)) {
-    case 16: launch_update_tile<V, 16>(Y, X, a, W1, W2, w2c, C, E, e, rows, L, s); break;
-    case 32: launch_update_tile<V, 32>(Y, X, a, W1, W2, w2c, C, E, e, rows, L, s); break;
-    default: launch_update_tile<V, 64>(Y, X, a, W1, W2, w2c, C, E, e, rows, L, s); break;
+    case 16: launch_update_tile<V, 16>(Y, X, a, W1, W2, w2c, C, E, e, rows, L, s, sc); break;
+    case 32: launch_update_tile<V, 32>(Y, X, a, W1, W2, w2c, C, E, e, rows, L, s, sc); break;
+    default: launch_update_tile<V, 64>(Y, X, a, W1, W2, w2c, C, E, e, rows, L, s, sc); break;
   }
 }
 template void launch_update<double2>(double2*, const double2*, double2, const double2*, const double2*, double2,
-                                     const double2*, const double2*, double2, long long, int, hipStream_t);
+                                     const double2*, const double2*, double2, long long, int, hipStream_t,
+                                     const double2*);
 template void launch_update<float2>(float2*, const float2*, double2, const float2*, const float2*, double2,
-                                    const double2*, const float2*, double2, long long, int, hipStream_t);
+                                    const double2*, const float2*, double2, long long, int, hipStream_t,
+                                    const double2*);
 
 // ------------------------------------------------------------ reductions
 constexpr int kDotParts = 1024;
@@ -498,8 +510,9 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 
 template <typename V>
 __global__ void __launch_bounds__(256) k_axpy_norm(V* Y, const V* X, double2 c, const V* Z, long long n,
-                                                   double* partial) {
+                                                   double* partial, const double2* c_dev) {
   using T = typename RealOf<V>::T;
+  if (c_dev) c = *c_dev;
   __shared__ double sh[4];
   double s = 0.0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
@@ -517,13 +530,13 @@ __global__ void __launch_bounds__(256) k_axpy_norm(V* Y, const V* X, double2 c, 
 
 template <typename V>
 void launch_axpy_norm(V* Y, const V* X, double2 c, const V* Z, long long n, double* partial, int nparts,
-                      hipStream_t s) {
-  hipLaunchKernelGGL((k_axpy_norm<V>), dim3(nparts), dim3(256), 0, s, Y, X, c, Z, n, partial);
+                      hipStream_t s, const double2* c_dev) {
+  hipLaunchKernelGGL((k_axpy_norm<V>), dim3(nparts), dim3(256), 0, s, Y, X, c, Z, n, partial, c_dev);
 }
 template void launch_axpy_norm<double2>(double2*, const double2*, double2, const double2*, long long, double*, int,
-                                        hipStream_t);
+                                        hipStream_t, const double2*);
 template void launch_axpy_norm<float2>(float2*, const float2*, double2, const float2*, long long, double*, int,
-                                       hipStream_t);
+                                       hipStream_t, const double2*);
 
 template <typename V>
 __global__ void __launch_bounds__(256) k_dot(const V* A, const V* B, long long n, double2* pdot, double* pnrm) {

@@ -126,6 +126,11 @@ struct bdda_plan {
   bdda::DevBuf blk[12];
   bdda::DevBuf gram_part, gram_out, coef, dots_d, dots_n, obs_part, Bj64;
   bdda::HostBuf h_gram, h_coef, h_dots;
+  // device L x L algebra: coefficient slots, and err / status / scalars /
+  // pivots, with the pinned host copy of err and status read once an
+  // iteration
+  bdda::DevBuf dla, dla_misc;
+  bdda::HostBuf h_sync;
   // optional per-pass kernel timing (HIP events recorded around each pass)
   bool profiling = false;
   std::vector<hipEvent_t> ev_pool;
@@ -807,9 +812,286 @@ static int solve_cocg(Ctx& c, const V* Bj, V* X, bool x_zero, double bnorm, doub
   return 0;
 }
 
+// ------------------------------------------- solvers with device L x L algebra
+// For L <= kDlaMax the recurrence coefficients are computed on the device
+// (kernels_dla.hip): Gram matrices feed the L x L solves directly, the
+// block updates read their coefficients (and block-BiCGSTAB's scalar xi)
+// from device memory, and an iteration waits on the host once, for its
+// residual and a status word (singular system, failed or shifted Cholesky).
+// Same recurrences, same step order and stopping rule as the host-algebra
+// solvers above; the L x L arithmetic differs from the host's only in
+// rounding.  Measured on the MI355X box (64^3 workload, L = 64: 38.9 against
+// 38.0 ms per float64 solve; the 32-particle batch at L = 16: equal within
+// 2%): the single-workgroup L x L kernels are latency-bound and take about
+// as long as the host's AVX-512 routines plus the waits they save, so the
+// host algebra stays the default and this path is selected with
+// bdda_set_la_path(1) (one wait per iteration; a host with a slower CPU or
+// a caller that wants no host work inside the iteration).
+std::atomic<int> g_la_path{0};  // 0: host algebra, 1: device algebra when L <= kDlaMax
+
+enum DlaSlot {
+  D_QTQ, D_STAS, D_ALPHA, D_NALPHA, D_AX, D_G, D_R1, D_R1I, D_R2, D_R2I, D_R3, D_R3I, D_TAU, D_TAU2, D_XI, D_XIP,
+  D_QNQN, D_RHS, D_BETA, D_LU, D_RV, D_RR, D_RZ, kDlaSlots
+};
+struct DlaMisc {  // layout of plan->dla_misc
+  double err;
+  int status;
+  int pad;
+  double2 sc[2];  // -xi, xi
+  int piv[kDlaMax];
+};
+
+struct Dla {
+  double2* slot[kDlaSlots];
+  DlaMisc* misc;
+  DlaMisc* host;
+};
+
+static bool use_dla(const Ctx& c) { return g_la_path.load() == 1 && c.L <= kDlaMax; }
+
+static int dla_setup(Ctx& c, Dla& d) {
+  bdda_plan* p = c.p;
+  const size_t one = (size_t)c.L * c.L;
+  int rc = p->dla.ensure(kDlaSlots * one * sizeof(double2));
+  if (!rc) rc = p->dla_misc.ensure(sizeof(DlaMisc));
+  if (!rc) rc = p->h_sync.ensure(sizeof(DlaMisc));
+  if (!rc) rc = p->gram_part.ensure((size_t)gram_parts() * ((c.L + 63) / 64) * ((c.L + 63) / 64) * 4096 *
+                                    sizeof(double2));
+  if (!rc) rc = p->dots_d.ensure(dot_parts() * sizeof(double2));
+  if (!rc) rc = p->dots_n.ensure(dot_parts() * sizeof(double));
+  if (rc) return rc;
+  for (int i = 0; i < kDlaSlots; ++i) d.slot[i] = p->dla.as<double2>() + i * one;
+  d.misc = p->dla_misc.as<DlaMisc>();
+  d.host = p->h_sync.as<DlaMisc>();
+  BDDA_HIP(hipMemsetAsync(&d.misc->status, 0, sizeof(int), c.s));
+  return 0;
+}
+
+static int gram_dev(Ctx& c, const double2* A, const double2* B, bool conj, double2* out) {
+  launch_gram<double2>(A, B, c.rows, c.L, conj, c.p->gram_part.as<double2>(), gram_parts(), out, c.s);
+  BDDA_HIP(hipGetLastError());
+  return 0;
+}
+
+// The iteration's one wait: err and status to the host.
+static int dla_sync(Ctx& c, Dla& d, double* err, int* status) {
+  BDDA_HIP(hipMemcpyAsync(d.host, d.misc, 2 * sizeof(double), hipMemcpyDeviceToHost, c.s));
+  BDDA_HIP(hipStreamSynchronize(c.s));
+  *err = d.host->err;
+  *status = d.host->status;
+  return 0;
+}
+
+// Y = a X + W1 C with C on the device (coefficient slot).
+static int update_dev(Ctx& c, double2* Y, const double2* X, zc a, const double2* W1, const double2* W2,
+                      const double2* C, const double2* E, const double2* sc) {
+  launch_update<double2>(Y, X, make_double2(a.real(), a.imag()), W1, W2, make_double2(0.0, 0.0), C, E,
+                         make_double2(0.0, 0.0), c.rows, c.L, c.s, sc);
+  BDDA_HIP(hipGetLastError());
+  return 0;
+}
+
+// Cholesky QR2 of W into Qn with tau = R2 R1 (device): the first pass
+// shifts its Gram matrix if the plain factorisation fails (status
+// DLA_SHIFTED; the caller then runs dla_qr_third_pass).
+static int cholqr2_dev(Ctx& c, Dla& d, const double2* W, double2* Q1, double2* Qn) {
+  int* st = &d.misc->status;
+  if (int rc = gram_dev(c, W, W, true, d.slot[D_G])) return rc;
+  dla_chol_inv(d.slot[D_G], c.L, (double)c.rows, true, d.slot[D_R1], d.slot[D_R1I], st, c.s);
+  if (int rc = update_dev(c, Q1, nullptr, 0.0, W, nullptr, d.slot[D_R1I], nullptr, nullptr)) return rc;
+  if (int rc = gram_dev(c, Q1, Q1, true, d.slot[D_G])) return rc;
+  dla_chol_inv(d.slot[D_G], c.L, (double)c.rows, false, d.slot[D_R2], d.slot[D_R2I], st, c.s);
+  if (int rc = update_dev(c, Qn, nullptr, 0.0, Q1, nullptr, d.slot[D_R2I], nullptr, nullptr)) return rc;
+  dla_matmul(d.slot[D_R2], false, d.slot[D_R1], d.slot[D_TAU], c.L, c.s);
+  BDDA_HIP(hipGetLastError());
+  return 0;
+}
+
+// Shifted CholeskyQR3's third pass on Qn (after a shifted first pass):
+// Qn <- Qn R3^-1 (through Q1), tau <- R3 tau.
+static int dla_qr_third_pass(Ctx& c, Dla& d, double2* Q1, double2* Qn) {
+  int* st = &d.misc->status;
+  if (int rc = gram_dev(c, Qn, Qn, true, d.slot[D_G])) return rc;
+  dla_chol_inv(d.slot[D_G], c.L, (double)c.rows, false, d.slot[D_R3], d.slot[D_R3I], st, c.s);
+  if (int rc = update_dev(c, Q1, nullptr, 0.0, Qn, nullptr, d.slot[D_R3I], nullptr, nullptr)) return rc;
+  BDDA_HIP(hipMemcpyAsync(Qn, Q1, (size_t)c.rows * c.L * sizeof(double2), hipMemcpyDeviceToDevice, c.s));
+  dla_matmul(d.slot[D_R3], false, d.slot[D_TAU], d.slot[D_TAU2], c.L, c.s);
+  BDDA_HIP(hipMemcpyAsync(d.slot[D_TAU], d.slot[D_TAU2], (size_t)c.L * c.L * sizeof(double2),
+                          hipMemcpyDeviceToDevice, c.s));
+  return 0;
+}
+
+static int dla_status_error(int status, const char* lu_what) {
+  if (status & DLA_SINGULAR) return singular(lu_what);
+  if (status & DLA_NOT_PD) return singular("block QR (Cholesky of the Gram matrix)");
+  return 0;
+}
+
+// The mixed-precision check at the top of iteration k (as in the host-
+// algebra solvers): true when the complex64 cycle must stop.
+static int conv32_check(Ctx& c, const double2* X, const double2* Bj, double2* AS, double2* W, double bnorm,
+                        double err, double tol, int k, bool* stop) {
+  *stop = false;
+  if (!(c.conv32 && c.check_every > 0 && k > 0 && k % c.check_every == 0)) return 0;
+  *stop = !conv32_on_track(err, tol, k, c.check_every);
+  if (!*stop) {
+    if (int rc = apply_operator<double2>(c.p, X, AS, c.L, true, c.s)) return rc;
+    double rt2;
+    if (int rc = axpy_norm_host<double2>(c, W, Bj, zc(-1.0), AS, &rt2)) return rc;
+    *stop = !(std::sqrt(rt2) / bnorm <= 2.0 * err);
+  }
+  if (*stop) c.conv32 = false;
+  return 0;
+}
+
+// Block-COCG-rq (bl_krylov.py:58-118) with device L x L algebra.
+static int solve_cocg_dev(Ctx& c, const double2* Bj, double2* X, bool x_zero, double bnorm, double tol,
+                          int itermax, double2** ws, int* it_out, double* err_out) {
+  const size_t bytes = (size_t)c.rows * c.L * sizeof(double2);
+  const size_t lbytes = (size_t)c.L * c.L * sizeof(double2);
+  double2 *Q = ws[0], *S = ws[1], *AS = ws[2], *W = ws[3], *Q1 = ws[4], *Sn = ws[5], *R0 = ws[6];
+  Dla d;
+  int rc = dla_setup(c, d);
+  if (rc) return rc;
+  if (x_zero) {
+    BDDA_HIP(hipMemsetAsync(X, 0, bytes, c.s));
+    BDDA_HIP(hipMemcpyAsync(R0, Bj, bytes, hipMemcpyDeviceToDevice, c.s));
+  } else {
+    if ((rc = op<double2>(c, X, AS))) return rc;
+    launch_axpy_norm<double2>(R0, Bj, make_double2(-1.0, 0.0), AS, c.rows * c.L, nullptr, dot_parts(), c.s);
+    BDDA_HIP(hipGetLastError());
+  }
+  // [Q, xi] = qr(R0), S = Q, QtQ = Q^T Q
+  if ((rc = cholqr2_dev(c, d, R0, Q1, Q))) return rc;
+  double e0;
+  int st;
+  if ((rc = dla_sync(c, d, &e0, &st))) return rc;
+  if (st & DLA_SHIFTED) {
+    int st3;
+    BDDA_HIP(hipMemsetAsync(&d.misc->status, 0, sizeof(int), c.s));
+    if ((rc = dla_qr_third_pass(c, d, Q1, Q)) || (rc = dla_sync(c, d, &e0, &st3))) return rc;
+    st = (st & ~DLA_SHIFTED) | st3;
+  }
+  if ((rc = dla_status_error(st, "block QR (Cholesky of W^H W)"))) return rc;
+  BDDA_HIP(hipMemcpyAsync(d.slot[D_XI], d.slot[D_TAU], lbytes, hipMemcpyDeviceToDevice, c.s));
+  BDDA_HIP(hipMemcpyAsync(S, Q, bytes, hipMemcpyDeviceToDevice, c.s));
+  if ((rc = gram_dev(c, Q, Q, false, d.slot[D_QTQ]))) return rc;
+  int it = 0;
+  double err = INFINITY;
+  for (int k = 0; k < itermax; ++k) {
+    bool stop = false;
+    if ((rc = conv32_check(c, X, Bj, AS, W, bnorm, err, tol, k, &stop))) return rc;
+    if (stop) break;
+    if ((rc = op<double2>(c, S, AS))) return rc;
+    if ((rc = gram_dev(c, S, AS, false, d.slot[D_STAS]))) return rc;
+    dla_lu_solve(d.slot[D_STAS], d.slot[D_QTQ], c.L, c.L, false, d.slot[D_ALPHA], d.slot[D_NALPHA], nullptr,
+                 nullptr, &d.misc->status, c.s);
+    dla_matmul(d.slot[D_ALPHA], false, d.slot[D_XI], d.slot[D_AX], c.L, c.s);
+    // X = X + S (alpha xi) ; W = Q - AS alpha
+    if ((rc = update_dev(c, X, X, 1.0, S, nullptr, d.slot[D_AX], nullptr, nullptr))) return rc;
+    if ((rc = update_dev(c, W, Q, 1.0, AS, nullptr, d.slot[D_NALPHA], nullptr, nullptr))) return rc;
+    // [Qn, tau] = qr(W) into AS (free until the next product); xi = tau xi
+    double2* Qn = AS;
+    if ((rc = cholqr2_dev(c, d, W, Q1, Qn))) return rc;
+    BDDA_HIP(hipMemcpyAsync(d.slot[D_XIP], d.slot[D_XI], lbytes, hipMemcpyDeviceToDevice, c.s));
+    dla_matmul(d.slot[D_TAU], false, d.slot[D_XIP], d.slot[D_XI], c.L, c.s);
+    dla_frob(d.slot[D_XI], c.L, bnorm, &d.misc->err, c.s);
+    if ((rc = dla_sync(c, d, &err, &st))) return rc;
+    if ((st & DLA_SHIFTED) && !(st & (DLA_SINGULAR | DLA_NOT_PD))) {
+      int st3;
+      BDDA_HIP(hipMemsetAsync(&d.misc->status, 0, sizeof(int), c.s));
+      if ((rc = dla_qr_third_pass(c, d, Q1, Qn))) return rc;
+      dla_matmul(d.slot[D_TAU], false, d.slot[D_XIP], d.slot[D_XI], c.L, c.s);
+      dla_frob(d.slot[D_XI], c.L, bnorm, &d.misc->err, c.s);
+      if ((rc = dla_sync(c, d, &err, &st3))) return rc;
+      st = (st & ~DLA_SHIFTED) | st3;
+    }
+    if ((rc = dla_status_error(st, "block-COCG-rq (S^T A S)"))) return rc;
+    if (c.hist) c.hist->push_back(err);
+    it = k;
+    if (err < tol) break;
+    // beta = (Q^T Q)^-1 tau^T (Qn^T Qn) ; Sn = Qn + S beta ; Q <- Qn
+    if ((rc = gram_dev(c, Qn, Qn, false, d.slot[D_QNQN]))) return rc;
+    dla_matmul(d.slot[D_TAU], true, d.slot[D_QNQN], d.slot[D_RHS], c.L, c.s);
+    dla_lu_solve(d.slot[D_QTQ], d.slot[D_RHS], c.L, c.L, false, d.slot[D_BETA], nullptr, nullptr, nullptr,
+                 &d.misc->status, c.s);
+    if ((rc = update_dev(c, Sn, Qn, 1.0, S, nullptr, d.slot[D_BETA], nullptr, nullptr))) return rc;
+    std::swap(S, Sn);
+    BDDA_HIP(hipMemcpyAsync(Q, Qn, bytes, hipMemcpyDeviceToDevice, c.s));
+    BDDA_HIP(hipMemcpyAsync(d.slot[D_QTQ], d.slot[D_QNQN], lbytes, hipMemcpyDeviceToDevice, c.s));
+  }
+  *it_out = it;
+  *err_out = err;
+  return 0;
+}
+
+// Block-BiCGSTAB (bl_krylov.py:121-187) with device L x L algebra.
+static int solve_bicgstab_dev(Ctx& c, const double2* Bj, double2* X, bool x_zero, double bnorm, double tol,
+                              int itermax, double2** ws, int* it_out, double* err_out) {
+  const long long n = c.rows * c.L;
+  const size_t bytes = n * sizeof(double2);
+  double2 *R = ws[0], *P = ws[1], *R0 = ws[2], *Vb = ws[3], *T = ws[4], *Z = ws[5], *Pn = ws[6];
+  Dla d;
+  int rc = dla_setup(c, d);
+  if (rc) return rc;
+  if (x_zero) {
+    BDDA_HIP(hipMemsetAsync(X, 0, bytes, c.s));
+    BDDA_HIP(hipMemcpyAsync(R, Bj, bytes, hipMemcpyDeviceToDevice, c.s));
+  } else {
+    if ((rc = op<double2>(c, X, T))) return rc;
+    launch_axpy_norm<double2>(R, Bj, make_double2(-1.0, 0.0), T, n, nullptr, dot_parts(), c.s);
+    BDDA_HIP(hipGetLastError());
+  }
+  BDDA_HIP(hipMemcpyAsync(P, R, bytes, hipMemcpyDeviceToDevice, c.s));
+  BDDA_HIP(hipMemcpyAsync(R0, R, bytes, hipMemcpyDeviceToDevice, c.s));
+  const double2* sc = d.misc->sc;  // -xi, xi
+  int it = 0, st = 0;
+  double err = INFINITY;
+  for (int k = 0; k < itermax; ++k) {
+    bool stop = false;
+    if ((rc = conv32_check(c, X, Bj, T, Z, bnorm, err, tol, k, &stop))) return rc;
+    if (stop) break;
+    if ((rc = op<double2>(c, P, Vb))) return rc;
+    if ((rc = gram_dev(c, R0, Vb, true, d.slot[D_RV])) || (rc = gram_dev(c, R0, R, true, d.slot[D_RR]))) return rc;
+    dla_lu_solve(d.slot[D_RV], d.slot[D_RR], c.L, c.L, false, d.slot[D_ALPHA], d.slot[D_NALPHA], d.slot[D_LU],
+                 d.misc->piv, &d.misc->status, c.s);
+    // T = R - V alpha
+    if ((rc = update_dev(c, T, R, 1.0, Vb, nullptr, d.slot[D_NALPHA], nullptr, nullptr))) return rc;
+    if ((rc = op<double2>(c, T, Z))) return rc;
+    // xi = tr(Z^H T) / tr(Z^H Z) from one pass over T and Z
+    launch_dot<double2>(T, Z, n, c.p->dots_d.as<double2>(), c.p->dots_n.as<double>(), dot_parts(), c.s);
+    BDDA_HIP(hipGetLastError());
+    dla_xi(c.p->dots_d.as<double2>(), c.p->dots_n.as<double>(), dot_parts(), &d.misc->status, d.misc->sc, c.s);
+    // X = X + P alpha + xi T ; R = T - xi Z with its norm ; R0^H Z
+    if ((rc = update_dev(c, X, X, 1.0, P, nullptr, d.slot[D_ALPHA], T, sc))) return rc;
+    launch_axpy_norm<double2>(R, T, make_double2(0.0, 0.0), Z, n, c.p->dots_n.as<double>(), dot_parts(), c.s, sc);
+    BDDA_HIP(hipGetLastError());
+    if ((rc = gram_dev(c, R0, Z, true, d.slot[D_RZ]))) return rc;
+    dla_norm_err(c.p->dots_n.as<double>(), dot_parts(), bnorm, &d.misc->err, c.s);
+    if ((rc = dla_sync(c, d, &err, &st))) return rc;
+    if ((rc = dla_status_error(st, "block-BiCGSTAB (R0^H V)"))) return rc;
+    if (c.hist) c.hist->push_back(err);
+    it = k;
+    if (err < tol) break;
+    // beta = (R0^H V)^-1 (-R0^H Z) ; Pn = R + (P - xi V) beta
+    dla_lu_resolve(d.slot[D_LU], d.misc->piv, d.slot[D_RZ], c.L, c.L, true, d.slot[D_BETA], &d.misc->status, c.s);
+    if ((rc = update_dev(c, Pn, R, 1.0, P, Vb, d.slot[D_BETA], nullptr, sc))) return rc;
+    std::swap(P, Pn);
+  }
+  *it_out = it;
+  *err_out = err;
+  return 0;
+}
+
 template <typename V>
 static int run_solver(Ctx& c, int solver, const V* Bj, V* X, bool x_zero, double bnorm, double tol, int itermax,
                       V** ws, int* it, double* err) {
+  if constexpr (std::is_same<V, double2>::value) {
+    if (use_dla(c)) {
+      if (solver == BDDA_SOLVER_BICGSTAB) return solve_bicgstab_dev(c, Bj, X, x_zero, bnorm, tol, itermax, ws, it, err);
+      return solve_cocg_dev(c, Bj, X, x_zero, bnorm, tol, itermax, ws, it, err);
+    }
+  }
   if (solver == BDDA_SOLVER_BICGSTAB) return solve_bicgstab<V>(c, Bj, X, x_zero, bnorm, tol, itermax, ws, it, err);
   return solve_cocg<V>(c, Bj, X, x_zero, bnorm, tol, itermax, ws, it, err);
 }
@@ -863,6 +1145,15 @@ extern "C" {
 
 const char* bdda_last_error(void) { return g_err.c_str(); }
 int bdda_version(void) { return 100; }
+
+int bdda_set_la_path(int path) {
+  if (path < 0 || path > 1) {
+    set_error("bdda_set_la_path: path must be 0 (host algebra) or 1 (device algebra for L <= 64)");
+    return BDDA_E_ARG;
+  }
+  g_la_path = path;
+  return 0;
+}
 
 int bdda_set_fft_path(int path) {
   if (path < 0 || path > 2) {

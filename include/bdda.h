@@ -54,6 +54,14 @@ int bdda_version(void);
  * cross-check the kernels. */
 int bdda_set_fft_path(int path);
 
+/* Where the solvers' L x L recurrence algebra runs, for the whole process:
+ * 0 (default) on the host after every Gram matrix (LAPACK-order LU with
+ * partial pivoting, Cholesky, triangular inverse; 3-4 waits per iteration);
+ * 1 on the device for L <= 64 (the same routines in single-workgroup
+ * kernels; one wait per iteration, for the convergence test), on the host
+ * for larger L.  On the MI355X box both take about the same time. */
+int bdda_set_la_path(int path);
+
 /* Plan over an nx x ny x nz cuboid lattice: chooses the convolution FFT
  * lengths m >= 2n - 1 per axis and owns every device buffer of the path.
  * Replaces the shape/allocation part of fft_init (mvp_fft/mvp_fft.py:97-119). */

@@ -34,18 +34,27 @@ def small_batch():
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_batch_matches_single_solves(small_batch, precision):
-    from bdda_amd import workloads
+@pytest.mark.parametrize("la_path", [0, 1])
+def test_batch_matches_single_solves(small_batch, precision, la_path):
+    """Host (0) or device (1) L x L algebra: the batch equals one-at-a-time
+    solves bit for bit either way (each instance's device algebra works in
+    its own plan's buffers on its own stream)."""
+    from bdda_amd import _lib, workloads
     from bdda_amd.batch import solve_batch
-    say(f"batch of {len(small_batch)} uneven instances, {precision}")
-    dds = [workloads.make_dipoles(w, geometry=g, precision=precision) for w, g in small_batch]
-    assert len({tuple(dd._plan.m) for dd in dds}) > 1  # uneven FFT grids
-    ref = []
-    for dd in dds:
-        dd.solve_matrix_equation()
-        assert dd.converge and dd.true_residual < 1e-5
-        ref.append((dd.X_device.clone(), dd.iter_fin, list(dd.err_history)))
-    solve_batch(dds, nstreams=3)
+    say(f"batch of {len(small_batch)} uneven instances, {precision}, la path {la_path}")
+    lib = _lib.load()
+    _lib.check(lib.bdda_set_la_path(la_path))
+    try:
+        dds = [workloads.make_dipoles(w, geometry=g, precision=precision) for w, g in small_batch]
+        assert len({tuple(dd._plan.m) for dd in dds}) > 1  # uneven FFT grids
+        ref = []
+        for dd in dds:
+            dd.solve_matrix_equation()
+            assert dd.converge and dd.true_residual < 1e-5
+            ref.append((dd.X_device.clone(), dd.iter_fin, list(dd.err_history)))
+        solve_batch(dds, nstreams=3)
+    finally:
+        lib.bdda_set_la_path(0)
     for dd, (X, it, hist) in zip(dds, ref):
         assert torch.equal(dd.X_device, X)
         assert dd.iter_fin == it and dd.err_history == hist and dd.converge

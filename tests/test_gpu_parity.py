@@ -339,6 +339,45 @@ def test_solver_iterates_and_observables_match_oracle(solver, L):
     assert dd.true_residual < 1e-7
 
 
+@pytest.mark.parametrize("solver", ["cocg_rq", "bicgstab"])
+@pytest.mark.parametrize("L", [6, 24, 64])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_device_and_host_la_paths_agree(solver, L, precision):
+    """The L x L recurrence algebra on the host (default) and on the device
+    (bdda_set_la_path(1), L <= 64) run the same recurrences: same
+    iteration count at L = 6, the first residuals equal to rounding, the
+    solutions equal to the tolerance class (rounding grows along
+    ill-conditioned iterations at L = 24 and 64)."""
+    say(f"la paths {solver} L={L} {precision}")
+    from bdda_amd import _lib
+    lib = _lib.load()
+    prob = _problem(L=L)
+    out = {}
+    try:
+        for path in (0, 1):
+            _lib.check(lib.bdda_set_la_path(path))
+            out[path] = _gpu_dd(prob, solver, 1e-8, 80, precision=precision)
+    finally:
+        lib.bdda_set_la_path(0)
+    h, d = out[0], out[1]
+    assert h.converge and d.converge and d.true_residual < 1e-7
+    # the first iteration's L x L systems are already ill-conditioned at
+    # L = 24, 64 on ~500 dipoles: rounding differences reach ~1e-10 there
+    np.testing.assert_allclose(d.err_history[0], h.err_history[0], rtol=1e-10 if L <= 8 else 1e-7)
+    if L <= 8:
+        assert d.iter_fin == h.iter_fin
+        np.testing.assert_allclose(d.err_history, h.err_history, rtol=1e-6)
+        assert rel(d.X, h.X) < 1e-9
+    else:
+        assert abs(d.iter_fin - h.iter_fin) <= 1
+        assert rel(d.X, h.X) < 1e-6
+
+
+def test_la_path_rejects_unknown():
+    from bdda_amd import _lib
+    assert _lib.load().bdda_set_la_path(2) == _lib.E_ARG
+
+
 def test_block_mvp_and_krylov_dropins_match_oracle():
     say("block_mvp drop-in")
     from bl_krylov import bl_krylov as pk
@@ -445,8 +484,17 @@ def test_cocg_nearly_dependent_orientations():
     eu = np.array(prob[6], dtype=np.float64)
     eu[1] = eu[0] + 1e-9
     prob[6] = eu
+    from bdda_amd import _lib
+    lib = _lib.load()
+    try:  # the shifted QR on the device's and on the host's L x L algebra
+        _lib.check(lib.bdda_set_la_path(1))
+        dh = _gpu_dd(tuple(prob), "cocg_rq", 1e-9, 80)
+    finally:
+        lib.bdda_set_la_path(0)
+    assert dh.converge and dh.true_residual < 1e-8
     dd = _gpu_dd(tuple(prob), "cocg_rq", 1e-9, 80)
     assert dd.converge and dd.true_residual < 1e-8
+    assert rel(dd.X, dh.X) < 1e-7
     for l in (0, 1, 3):
         one = list(prob)
         one[6] = eu[l:l + 1]
