@@ -132,12 +132,15 @@ def test_fno_training_reduces_loss():
     assert losses[-1] < 0.75 * losses[0] and losses[-1] < min(losses[:10]), losses[::10]
 
 
-def test_trained_warm_start_gain():
+def test_trained_warm_start_no_regression():
     """The committed network (trained on 2000 solver-generated particles)
     against a zero start on held-out isotropic particles: mixed-precision
-    block-COCG-rq to 1e-5 at L = 64 orientations.  The guess must be taken,
-    reach the same true residual and cost at most one extra iteration; the
-    saved iterations are printed.  A guess worse than zero must be discarded
+    block-COCG-rq to 1e-5 at L = 64 orientations.  A no-regression check,
+    not a gain test: the network's guess saves -1 to +1 iterations on these
+    particles (and 2-3 of 143 on the 96^3 configuration, less than its
+    inference costs), so the test asserts that the guess is taken, reaches
+    the same true residual and costs at most one extra iteration; the saved
+    iterations are printed.  A guess worse than zero must be discarded
     (zero-start iteration count)."""
     say("trained warm start")
     import os
