@@ -12,8 +12,11 @@ host only when read).
 Extensions (attributes, set before ``solve_matrix_equation``):
 ``solver`` ('bicgstab' as the reference, or 'cocg_rq'), ``precision``
 ('fp64' or 'mixed'), ``refine_every`` (20), ``X0`` (warm start, (3N, L)),
-``pad_to`` (compute on a cuboid padded with vacuum cells to at least this
-many cells per axis; the physics is unchanged), ``verbose``.
+``pad_to`` (compute on the lattice padded with vacuum cells to at least
+this many cells per axis; without it the solver's cuboid is the smallest box
+of lattice cells holding the occupied sites, ``solver_dims`` after
+``set_interaction_matrix``; the physics is unchanged either way),
+``verbose``.
 """
 import time
 
@@ -117,6 +120,7 @@ class DiscreteDipoles(Target, IncidentField):
         self.check_true_residual = False
         self.X0 = None
         self.pad_to = None
+        self.solver_dims = None
         self.verbose = DiscreteDipoles.verbose_default
         self.iter_fin = None
         self.err_fin = None
@@ -132,11 +136,23 @@ class DiscreteDipoles(Target, IncidentField):
             print("Number of dipoles per wavelength in the particle volume: dpl= {:}".format(self.dpl))
 
     # ---------------------------------------------------------- set-up
-    def _compute_dims(self):
+    def _compute_dims(self, ix, iy, iz):
+        """Solver cuboid (cells per axis) and the lattice cell at its origin.
+
+        With ``pad_to`` the lattice itself, padded with vacuum cells to at
+        least ``pad_to``.  Without it the smallest box of lattice cells that
+        holds every occupied site: the enclosing lattice keeps a margin of
+        vacuum cells round the particle (up to a third of an axis), which
+        would only enlarge the FFT grid.  Either way the convolution over
+        the occupied sites is exact and the physics is unchanged."""
         n = np.array([int(v) for v in self.lattice_n])
         if self.pad_to is not None:
-            n = np.maximum(n, np.asarray(self.pad_to, dtype=int))
-        return n
+            return np.maximum(n, np.asarray(self.pad_to, dtype=int)), np.zeros(3, dtype=np.int64)
+        if ix.size == 0:
+            return n, np.zeros(3, dtype=np.int64)
+        lo = np.array([ix.min(), iy.min(), iz.min()], dtype=np.int64)
+        hi = np.array([ix.max(), iy.max(), iz.max()], dtype=np.int64) + 1
+        return hi - lo, lo
 
     def set_interaction_matrix(self):
         """Polarizability, diag(A), device Green spectrum and B (scatterer.py:135-161)."""
@@ -147,14 +163,15 @@ class DiscreteDipoles(Target, IncidentField):
         self.alpha_E = alpha0 / (1 - M_term * alpha0 / self.element_vol)
         self.diag_A = (1.0 / self.alpha_E).ravel()
         self.jpre = 1.0 / self.diag_A
-        dims = self._compute_dims()
         nx0, ny0, nz0 = (int(v) for v in self.lattice_n)
         addr = self.lattice_address_in_target.astype(np.int64)
+        ix, rem = np.divmod(addr, ny0 * nz0)
+        iy, iz = np.divmod(rem, nz0)
+        dims, lo = self._compute_dims(ix, iy, iz)
         if tuple(dims) != (nx0, ny0, nz0):
-            ix, rem = np.divmod(addr, ny0 * nz0)
-            iy, iz = np.divmod(rem, nz0)
-            addr = (ix * dims[1] + iy) * dims[2] + iz
+            addr = ((ix - lo[0]) * dims[1] + (iy - lo[1])) * dims[2] + (iz - lo[2])
         self._plan_address = addr
+        self.solver_dims = tuple(int(v) for v in dims)
         plan = Plan(dims)
         self.Au_til = GreenTensor(dims, self.lattice_lf, self.k, plan=plan)
         plan.set_target(addr, self.diag_A, self.jpre, pos=self.lattice_pos_in_target,

@@ -1292,11 +1292,16 @@ void launch_reg2(LinePass lp, hipStream_t s) {
 
 // Register-kernel lengths m = R1 * R2, in four groups (one per translation
 // unit).  R1 is even where the length allows, so the dedicated x pass and
-// its pruning apply.
-#define BDDA_REG2_LENGTHS_A(X) X(16, 4, 4) X(30, 6, 5) X(45, 9, 5) X(60, 12, 5) X(90, 10, 9) X(128, 16, 8) X(180, 12, 15) X(256, 16, 16)
-#define BDDA_REG2_LENGTHS_B(X) X(20, 4, 5) X(32, 8, 4) X(48, 16, 3) X(72, 12, 6) X(100, 10, 10) X(135, 15, 9) X(192, 16, 12)
-#define BDDA_REG2_LENGTHS_C(X) X(24, 8, 3) X(36, 12, 3) X(54, 6, 9) X(75, 15, 5) X(96, 16, 6) X(120, 12, 10) X(144, 16, 9) X(225, 15, 15)
-#define BDDA_REG2_LENGTHS_D(X) X(40, 8, 5) X(50, 10, 5) X(64, 8, 8) X(80, 16, 5) X(81, 9, 9) X(108, 12, 9) X(150, 10, 15) X(160, 16, 10) X(240, 16, 15)
+// its pruning apply.  A column gets max(R1, R2) threads, of which R2 work in
+// the first step and R1 in the second, so the pairs are as even as the
+// radix set allows: 24, 36, 48, 60, 72, 80 and 96 as (8,3), (12,3), (16,3),
+// (12,5), (12,6), (16,5), (16,6) ran the whole-cuboid operator 1.0-1.7x
+// slower than with the pairs below (complex128, L = 64, m^3 grids).  128
+// keeps (16,8) and 144 keeps (16,9): (8,16) and (12,12) measured slower.
+#define BDDA_REG2_LENGTHS_A(X) X(16, 4, 4) X(30, 6, 5) X(45, 9, 5) X(60, 10, 6) X(90, 10, 9) X(128, 16, 8) X(180, 12, 15) X(256, 16, 16)
+#define BDDA_REG2_LENGTHS_B(X) X(20, 4, 5) X(32, 8, 4) X(48, 8, 6) X(72, 8, 9) X(100, 10, 10) X(135, 15, 9) X(192, 16, 12)
+#define BDDA_REG2_LENGTHS_C(X) X(24, 6, 4) X(36, 6, 6) X(54, 6, 9) X(75, 15, 5) X(96, 12, 8) X(120, 12, 10) X(144, 16, 9) X(225, 15, 15)
+#define BDDA_REG2_LENGTHS_D(X) X(40, 8, 5) X(50, 10, 5) X(64, 8, 8) X(80, 10, 8) X(81, 9, 9) X(108, 12, 9) X(150, 10, 15) X(160, 16, 10) X(240, 16, 15)
 #define BDDA_REG2_LENGTHS(X) BDDA_REG2_LENGTHS_A(X) BDDA_REG2_LENGTHS_B(X) BDDA_REG2_LENGTHS_C(X) BDDA_REG2_LENGTHS_D(X)
 
 #define BDDA_REG2_INSTANTIATE(m, r1, r2)                                \

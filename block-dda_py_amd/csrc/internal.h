@@ -35,6 +35,11 @@ struct AxisPlan {
 // Radix sequence for a 5-smooth length (radices 8, 4, 2, 3, 5).
 bool factor_length(int m, AxisPlan& ax);
 int smallest_5smooth_at_least(int v);
+// Convolution FFT length for n cells on axis a (0 = x): 2n when 5-smooth,
+// else the smallest 5-smooth length >= 2n - 1, except on x, where it is the
+// smallest register-kernel length >= 2n - 1 with an even first radix (what
+// the dedicated x-pass kernel k_xconv needs) when one exists.
+int conv_length(int n, int axis);
 
 // Launch-time description of one batched line-FFT pass.  Every launch moves
 // `npts` points (lines) x Q columns of length-m transforms through LDS.

@@ -208,6 +208,28 @@ int smallest_5smooth_at_least(int v) {
   }
 }
 
+// Register-kernel lengths and their first radix.
+static const int kRegLengths[][2] = {
+#define BDDA_REG_ENTRY(m, r1, r2) {m, r1},
+    BDDA_REG2_LENGTHS(BDDA_REG_ENTRY)
+#undef BDDA_REG_ENTRY
+};
+
+int conv_length(int n, int axis) {
+  const int two = 2 * n;
+  if (smallest_5smooth_at_least(two) == two) return two;
+  const int lo = 2 * n - 1;
+  const int m = smallest_5smooth_at_least(lo);
+  if (axis != 0) return m;
+  // x: the dedicated x-pass kernel (k_xconv) needs a register length with an even
+  // first radix; take the smallest such length >= 2N - 1 when there is one
+  // (45, 75, 81, 135 and 225 become 48, 80, 90, 144 and 240)
+  int best = 0;
+  for (const auto& e : kRegLengths)
+    if (e[0] >= lo && e[1] % 2 == 0 && (best == 0 || e[0] < best)) best = e[0];
+  return best ? best : m;
+}
+
 bool factor_length(int m, AxisPlan& ax) {
   ax.m = m;
   ax.h = m / 2 + 1;

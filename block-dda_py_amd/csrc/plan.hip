@@ -1174,10 +1174,9 @@ int bdda_plan_create(int nx, int ny, int nz, bdda_plan** out) {
   const int n[3] = {nx, ny, nz};
   for (int a = 0; a < 3; ++a) {
     p->ax[a].n = n[a];
-    int m = 2 * n[a];
-    // keep the reference's 2N grid when it is 5-smooth, else the smallest
-    // 5-smooth length >= 2N - 1 (still an exact linear convolution)
-    if (smallest_5smooth_at_least(m) != m) m = smallest_5smooth_at_least(2 * n[a] - 1);
+    // the reference's 2N grid when it is 5-smooth, else a length >= 2N - 1
+    // (still an exact linear convolution)
+    int m = conv_length(n[a], a);
     if (m < 2) m = 2;
     if (!factor_length(m, p->ax[a])) {
       set_error("bdda_plan_create: cannot factor FFT length");
