@@ -216,5 +216,81 @@ double frob(const std::vector<zc>& A) {
   return std::sqrt(s);
 }
 
+namespace {
+// w = v^H y over entries [j, rows) with v[j] = 1 (the reflector's head)
+inline zc reflector_dot(const double* vr, const double* vi, const double* yr, const double* yi, long long j,
+                        long long rows) {
+  double wr = yr[j], wi = yi[j];
+  for (long long r = j + 1; r < rows; ++r) {
+    wr += vr[r] * yr[r] + vi[r] * yi[r];
+    wi += vr[r] * yi[r] - vi[r] * yr[r];
+  }
+  return zc(wr, wi);
+}
+// y -= f v over entries [j, rows) with v[j] = 1
+inline void reflector_axpy(const double* vr, const double* vi, double* yr, double* yi, zc f, long long j,
+                           long long rows) {
+  yr[j] -= f.real();
+  yi[j] -= f.imag();
+  split_axpy_neg(yr + j + 1, yi + j + 1, f.real(), f.imag(), vr + j + 1, vi + j + 1, (int)(rows - j - 1));
+}
+}  // namespace
+
+// zgeqr2: for each column j the reflector H_j = I - tau_j v v^H (zlarfg:
+// beta = -sign(Re a_jj) |a_j[j:]|, tau_j = (beta - a_jj) / beta, v = a_j[j:] /
+// (a_jj - beta) with v_j = 1) maps a_j[j:] to beta e_j; H_j^H is applied to
+// the later columns.  zung2r then forms Q = H_0 ... H_{n-1} [I; 0] from the
+// last reflector back.
+BDDA_HOST_CLONES void householder_qr(std::vector<double>& re, std::vector<double>& im, long long rows, int n,
+                                     std::vector<zc>& R) {
+  std::vector<zc> tau(n, zc(0.0));
+  for (int j = 0; j < n; ++j) {
+    double* vr = re.data() + (size_t)j * rows;
+    double* vi = im.data() + (size_t)j * rows;
+    const zc alpha(vr[j], vi[j]);
+    double s = 0.0;
+    for (long long r = j + 1; r < rows; ++r) s += vr[r] * vr[r] + vi[r] * vi[r];
+    if (s == 0.0 && alpha.imag() == 0.0) continue;  // H_j = I
+    double beta = std::sqrt(std::norm(alpha) + s);
+    if (alpha.real() >= 0.0) beta = -beta;
+    tau[j] = zc((beta - alpha.real()) / beta, -alpha.imag() / beta);
+    const zc scale = 1.0 / (alpha - beta);
+    for (long long r = j + 1; r < rows; ++r) {
+      const zc v = mul(zc(vr[r], vi[r]), scale);
+      vr[r] = v.real();
+      vi[r] = v.imag();
+    }
+    vr[j] = beta;
+    vi[j] = 0.0;
+    for (int k = j + 1; k < n; ++k) {
+      double* yr = re.data() + (size_t)k * rows;
+      double* yi = im.data() + (size_t)k * rows;
+      const zc f = mul(std::conj(tau[j]), reflector_dot(vr, vi, yr, yi, j, rows));
+      reflector_axpy(vr, vi, yr, yi, f, j, rows);
+    }
+  }
+  R.assign((size_t)n * n, zc(0.0));
+  for (int i = 0; i < n; ++i)
+    for (int l = i; l < n; ++l) R[(size_t)i * n + l] = zc(re[(size_t)l * rows + i], im[(size_t)l * rows + i]);
+  for (int j = n - 1; j >= 0; --j) {
+    double* vr = re.data() + (size_t)j * rows;
+    double* vi = im.data() + (size_t)j * rows;
+    for (int k = j + 1; k < n; ++k) {
+      double* yr = re.data() + (size_t)k * rows;
+      double* yi = im.data() + (size_t)k * rows;
+      const zc f = mul(tau[j], reflector_dot(vr, vi, yr, yi, j, rows));
+      reflector_axpy(vr, vi, yr, yi, f, j, rows);
+    }
+    for (long long r = j + 1; r < rows; ++r) {
+      const zc v = mul(zc(vr[r], vi[r]), -tau[j]);
+      vr[r] = v.real();
+      vi[r] = v.imag();
+    }
+    vr[j] = 1.0 - tau[j].real();
+    vi[j] = -tau[j].imag();
+    for (long long r = 0; r < j; ++r) vr[r] = vi[r] = 0.0;
+  }
+}
+
 }  // namespace la
 }  // namespace bdda

@@ -1,8 +1,9 @@
 // Small dense complex float64 linear algebra on the host for the L x L
-// recurrence coefficients of the block-Krylov solvers (L <= 128): LU with
-// partial pivoting (the numpy.linalg.solve / LAPACK gesv algorithm),
-// Cholesky of a Hermitian positive definite Gram matrix, triangular inverse
-// and small products.  Row-major storage; implemented in host_la.cpp.
+// recurrence coefficients of the block-Krylov solvers: LU with partial
+// pivoting (the numpy.linalg.solve / LAPACK gesv algorithm), Cholesky of a
+// Hermitian positive definite Gram matrix, triangular inverse and small
+// products (row-major storage), and a Householder QR of a tall block, the
+// block QR's last resort.  Implemented in host_la.cpp.
 #pragma once
 #include <complex>
 #include <vector>
@@ -24,6 +25,13 @@ void upper_inverse(const std::vector<zc>& R, std::vector<zc>& Ri, int n);
 // C = op(A) B  with op = transpose (no conjugate) when ta.
 void matmul(const std::vector<zc>& A, bool ta, const std::vector<zc>& B, std::vector<zc>& C, int n);
 double frob(const std::vector<zc>& A);
+// Reduced Householder QR of a rows x n block held as split column-major
+// arrays (column l at l * rows): on return they hold Q (orthonormal
+// columns) and R (n x n, row-major, upper triangular, real diagonal).
+// LAPACK's zgeqrf / zungqr reflectors, as numpy.linalg.qr; it needs no
+// positive definite Gram matrix, so it also factors a numerically
+// rank-deficient block.
+void householder_qr(std::vector<double>& re, std::vector<double>& im, long long rows, int n, std::vector<zc>& R);
 
 }  // namespace la
 }  // namespace bdda

@@ -114,7 +114,7 @@ void launch_green_expand(const double2* half, double2* full, const AxisPlan* ax,
 template <typename V>
 void launch_gram(const V* A, const V* B, long long rows, int L, bool conjA, double2* partial,
                  int nparts, double2* out, hipStream_t s);
-int gram_parts();
+int gram_parts(int L);
 // Y = a*X + sum_j (W1 + w2c*W2)[i][j] C[j][l] + e*E ;  X, W2, E nullable.
 template <typename V>
 void launch_update(V* Y, const V* X, double2 a, const V* W1, const V* W2, double2 w2c,
@@ -168,6 +168,10 @@ void launch_rhs(double2* B, const double* pos, const double* u, const double2* e
 void launch_observables(const double2* X, const double* pos, const double* geo, const double2* epsfac,
                         double k, long long N, int L, double2* partial, int nparts, hipStream_t s);
 int obs_parts();
+
+// Largest block width the solver entry points take.  The algorithms have no
+// limit; this one bounds the L x L host algebra and the Gram workspace.
+constexpr int kMaxL = 4096;
 
 // --------------------------------------------- device L x L algebra (L <= 64)
 constexpr int kDlaMax = 64;

@@ -4,6 +4,7 @@
 // Krylov blocks are (rows, L) row-major with rows = 3N (row = 3m + c) and the
 // L orientation columns innermost, so a wavefront reads 64 consecutive
 // columns of one row.  Small L x L coefficient matrices are complex float64.
+#include <algorithm>
 #include <mutex>
 #include <type_traits>
 
@@ -32,8 +33,14 @@ __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
 
 // --------------------------------------------------------------------- Gram
 // partial[wg][tile](64x64) = sum_{rows of wg} op(A[i][jt*64+a]) * B[i][lt*64+b]
+// Row ranges (parts) per Gram: 512 up to L = 128; beyond, fewer, so that
+// the partial tiles (parts x ceil(L/64)^2 x 64 x 64) stay near 64 MB while
+// the grid keeps at least 512 blocks.
 constexpr int kGramParts = 512;
-int gram_parts() { return kGramParts; }
+int gram_parts(int L) {
+  const int nt = (L + 63) / 64;
+  return nt <= 2 ? kGramParts : std::max(8, 4 * kGramParts / (nt * nt));
+}
 
 // A block owns a contiguous range of rows and one TL x TL output tile
 // (TL = 64, 32 or 16: the smallest that holds L, so that small L do not

@@ -445,7 +445,7 @@ static int gram_launch(Ctx& c, const void* A, const void* B, bool conj, bool f32
   bdda_plan* p = c.p;
   const int L = c.L;
   const int ntile = (L + 63) / 64;
-  const int np = gram_parts();
+  const int np = gram_parts(L);
   const size_t one = (size_t)L * L * sizeof(double2);
   int rc = p->gram_part.ensure((size_t)np * ntile * ntile * 4096 * sizeof(double2));
   if (!rc) rc = p->gram_out.ensure(kGramSlots * one);
@@ -683,6 +683,37 @@ static int solve_bicgstab(Ctx& c, const V* Bj, V* X, bool x_zero, double bnorm, 
   return 0;
 }
 
+// The block QR's last resort: Householder QR of W on the host (LAPACK's
+// reflectors, as numpy.linalg.qr in the reference), Q into Qn, R into tau.
+// Taken when even shifted CholeskyQR3 breaks down, i.e. W is numerically
+// rank-deficient, as a block of more orientations than the particle's field
+// has independent modes becomes; the reference's QR proceeds there too.
+template <typename V>
+static int householder_host(Ctx& c, const V* W, V* Qn, std::vector<zc>& tau) {
+  const long long rows = c.rows;
+  const int L = c.L;
+  const size_t n = (size_t)rows * L;
+  std::vector<V> h(n);
+  BDDA_HIP(hipMemcpyAsync(h.data(), W, n * sizeof(V), hipMemcpyDeviceToHost, c.s));
+  BDDA_HIP(hipStreamSynchronize(c.s));
+  std::vector<double> re(n), im(n);
+  for (long long r = 0; r < rows; ++r)
+    for (int l = 0; l < L; ++l) {
+      re[(size_t)l * rows + r] = h[(size_t)r * L + l].x;
+      im[(size_t)l * rows + r] = h[(size_t)r * L + l].y;
+    }
+  la::householder_qr(re, im, rows, L, tau);
+  using T = decltype(h[0].x);
+  for (long long r = 0; r < rows; ++r)
+    for (int l = 0; l < L; ++l) {
+      h[(size_t)r * L + l].x = (T)re[(size_t)l * rows + r];
+      h[(size_t)r * L + l].y = (T)im[(size_t)l * rows + r];
+    }
+  BDDA_HIP(hipMemcpyAsync(Qn, h.data(), n * sizeof(V), hipMemcpyHostToDevice, c.s));
+  BDDA_HIP(hipStreamSynchronize(c.s));
+  return 0;
+}
+
 // Reduced QR of W by Cholesky-QR2 in float64: Qn (device) and tau (host).
 template <typename V>
 static int cholqr2(Ctx& c, const V* W, V* Q1, V* Qn, std::vector<zc>& tau) {
@@ -716,13 +747,11 @@ static int cholqr2(Ctx& c, const V* W, V* Q1, V* Qn, std::vector<zc>& tau) {
   }
   // Nearly rank-deficient block (the Gram matrix squares its condition
   // number): shifted CholeskyQR3 from W -- a shifted pass, then two plain
-  // passes that restore orthogonality.
+  // passes that restore orthogonality; W is left intact for the last resort.
   if ((rc = pass(W, Q1, R1, true, 3, &ok1))) return rc;
-  if (!ok1) return singular("block QR (Cholesky of W^H W)");
-  if ((rc = pass(Q1, Qn, R2, false, 4, &ok2))) return rc;
-  if (!ok2) return singular("block QR (Cholesky of Q1^H Q1)");
-  if ((rc = pass(Qn, Q1, R3, false, 4, &ok3))) return rc;
-  if (!ok3) return singular("block QR (Cholesky of Q2^H Q2)");
+  if (ok1 && (rc = pass(Q1, Qn, R2, false, 4, &ok2))) return rc;
+  if (ok1 && ok2 && (rc = pass(Qn, Q1, R3, false, 4, &ok3))) return rc;
+  if (!(ok1 && ok2 && ok3)) return householder_host<V>(c, W, Qn, tau);
   BDDA_HIP(hipMemcpyAsync(Qn, Q1, (size_t)c.rows * L * sizeof(V), hipMemcpyDeviceToDevice, c.s));
   la::matmul(R2, false, R1, t, L);
   la::matmul(R3, false, t, tau, L);
@@ -855,7 +884,7 @@ static int dla_setup(Ctx& c, Dla& d) {
   int rc = p->dla.ensure(kDlaSlots * one * sizeof(double2));
   if (!rc) rc = p->dla_misc.ensure(sizeof(DlaMisc));
   if (!rc) rc = p->h_sync.ensure(sizeof(DlaMisc));
-  if (!rc) rc = p->gram_part.ensure((size_t)gram_parts() * ((c.L + 63) / 64) * ((c.L + 63) / 64) * 4096 *
+  if (!rc) rc = p->gram_part.ensure((size_t)gram_parts(c.L) * ((c.L + 63) / 64) * ((c.L + 63) / 64) * 4096 *
                                     sizeof(double2));
   if (!rc) rc = p->dots_d.ensure(dot_parts() * sizeof(double2));
   if (!rc) rc = p->dots_n.ensure(dot_parts() * sizeof(double));
@@ -868,7 +897,7 @@ static int dla_setup(Ctx& c, Dla& d) {
 }
 
 static int gram_dev(Ctx& c, const double2* A, const double2* B, bool conj, double2* out) {
-  launch_gram<double2>(A, B, c.rows, c.L, conj, c.p->gram_part.as<double2>(), gram_parts(), out, c.s);
+  launch_gram<double2>(A, B, c.rows, c.L, conj, c.p->gram_part.as<double2>(), gram_parts(c.L), out, c.s);
   BDDA_HIP(hipGetLastError());
   return 0;
 }
@@ -918,6 +947,18 @@ static int dla_qr_third_pass(Ctx& c, Dla& d, double2* Q1, double2* Qn) {
   dla_matmul(d.slot[D_R3], false, d.slot[D_TAU], d.slot[D_TAU2], c.L, c.s);
   BDDA_HIP(hipMemcpyAsync(d.slot[D_TAU], d.slot[D_TAU2], (size_t)c.L * c.L * sizeof(double2),
                           hipMemcpyDeviceToDevice, c.s));
+  return 0;
+}
+
+// The Householder last resort on the device-algebra path: Qn from W on the
+// host, tau into its slot, the status cleared (called only without
+// DLA_SINGULAR, which must survive).
+static int householder_dev(Ctx& c, Dla& d, const double2* W, double2* Qn) {
+  std::vector<zc> tau;
+  if (int rc = householder_host<double2>(c, W, Qn, tau)) return rc;
+  BDDA_HIP(hipMemcpyAsync(d.slot[D_TAU], tau.data(), tau.size() * sizeof(zc), hipMemcpyHostToDevice, c.s));
+  BDDA_HIP(hipMemsetAsync(&d.misc->status, 0, sizeof(int), c.s));
+  BDDA_HIP(hipStreamSynchronize(c.s));
   return 0;
 }
 
@@ -972,6 +1013,10 @@ static int solve_cocg_dev(Ctx& c, const double2* Bj, double2* X, bool x_zero, do
     if ((rc = dla_qr_third_pass(c, d, Q1, Q)) || (rc = dla_sync(c, d, &e0, &st3))) return rc;
     st = (st & ~DLA_SHIFTED) | st3;
   }
+  if ((st & DLA_NOT_PD) && !(st & DLA_SINGULAR)) {
+    if ((rc = householder_dev(c, d, R0, Q))) return rc;
+    st = 0;
+  }
   if ((rc = dla_status_error(st, "block QR (Cholesky of W^H W)"))) return rc;
   BDDA_HIP(hipMemcpyAsync(d.slot[D_XI], d.slot[D_TAU], lbytes, hipMemcpyDeviceToDevice, c.s));
   BDDA_HIP(hipMemcpyAsync(S, Q, bytes, hipMemcpyDeviceToDevice, c.s));
@@ -1005,6 +1050,12 @@ static int solve_cocg_dev(Ctx& c, const double2* Bj, double2* X, bool x_zero, do
       dla_frob(d.slot[D_XI], c.L, bnorm, &d.misc->err, c.s);
       if ((rc = dla_sync(c, d, &err, &st3))) return rc;
       st = (st & ~DLA_SHIFTED) | st3;
+    }
+    if ((st & DLA_NOT_PD) && !(st & DLA_SINGULAR)) {
+      if ((rc = householder_dev(c, d, W, Qn))) return rc;
+      dla_matmul(d.slot[D_TAU], false, d.slot[D_XIP], d.slot[D_XI], c.L, c.s);
+      dla_frob(d.slot[D_XI], c.L, bnorm, &d.misc->err, c.s);
+      if ((rc = dla_sync(c, d, &err, &st))) return rc;
     }
     if ((rc = dla_status_error(st, "block-COCG-rq (S^T A S)"))) return rc;
     if (c.hist) c.hist->push_back(err);
@@ -1481,10 +1532,10 @@ int bdda_solve(bdda_plan* p, int solver, int precision, const void* B_dev, const
     set_error("bdda_solve: call bdda_green_init and bdda_set_target first");
     return BDDA_E_STATE;
   }
-  if (L < 1 || L > 128 || !B_dev || !X_dev || itermax < 0 ||
+  if (L < 1 || L > kMaxL || !B_dev || !X_dev || itermax < 0 ||
       (solver != BDDA_SOLVER_COCG_RQ && solver != BDDA_SOLVER_BICGSTAB) ||
       (precision != BDDA_PREC_FP64 && precision != BDDA_PREC_MIXED)) {
-    set_error("bdda_solve: bad arguments (1 <= L <= 128, known solver and precision)");
+    set_error("bdda_solve: bad arguments (1 <= L <= 4096, known solver and precision)");
     return BDDA_E_ARG;
   }
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
@@ -1744,8 +1795,8 @@ int bdda_project_guess(bdda_plan* p, const void* B_dev, void* X0_dev, int L, dou
     set_error("bdda_project_guess: call bdda_green_init and bdda_set_target first");
     return BDDA_E_STATE;
   }
-  if (L < 1 || L > 128 || !B_dev || !X0_dev) {
-    set_error("bdda_project_guess: bad arguments (1 <= L <= 128, B and X0 on the device)");
+  if (L < 1 || L > kMaxL || !B_dev || !X0_dev) {
+    set_error("bdda_project_guess: bad arguments (1 <= L <= 4096, B and X0 on the device)");
     return BDDA_E_ARG;
   }
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);

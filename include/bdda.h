@@ -152,7 +152,8 @@ int bdda_block_mvp(bdda_plan* plan, const void* S_dev, void* AS_dev, int L, void
  *                 solve fell back to complex128 cycles; bits 2-7 (mixed):
  *                 refinement cycles run (at most 63); bits 8 and up: the
  *                 number of err_hist entries written (may be NULL)
- * 1 <= L <= 128.
+ * 1 <= L <= 4096 (the reference has no limit; larger blocks can be solved
+ * in column groups).
  * Non-convergence is not an error: err_fin >= tol, as in the reference.
  * Replaces bl_cocg_rq_jacobi_mvp_fft and bl_bicgstab_jacobi_mvp_fft
  * (bl_krylov/bl_krylov.py:58-187). */

@@ -313,6 +313,36 @@ def _gpu_dd(prob, solver="bicgstab", tol=1e-2, itermax=25, precision="fp64", **k
 
 
 @pytest.mark.parametrize("solver", ["cocg_rq", "bicgstab"])
+@pytest.mark.parametrize("L", [150, 200])
+def test_wide_blocks_match_oracle(solver, L):
+    """More orientations than 128 in one block solve (the reference has no
+    limit): three and four 64-column Gram tiles with fewer row parts, the
+    generic update kernel.  As for L = 70, ill-conditioned L x L systems
+    let rounding differences grow, so the converged solutions agree to the
+    tolerance rather than iterate for iterate."""
+    say(f"wide block {solver} L={L}")
+    prob = _problem(r=0.12, L=L)
+    ref = _oracle_solve(prob, "cocg" if solver == "cocg_rq" else "bicgstab", 1e-8, 80)
+    dd = _gpu_dd(prob, solver, 1e-8, 80)
+    assert dd.converge and abs(dd.iter_fin - ref["iter_fin"]) <= 2
+    assert rel(dd.X, ref["X"]) < 1e-6 and dd.true_residual < 1e-7
+    np.testing.assert_allclose(dd.compute_C_ext(), ref["C_ext"], rtol=1e-6)
+    sb = dd.compute_OCBS_observable_S_bk()
+    assert np.max(np.abs(sb - ref["S_bk"]) / np.abs(ref["S_bk"])) < 1e-6
+
+
+def test_block_width_limit_is_reported():
+    from bdda_amd.plan import Plan
+    p = Plan((4, 4, 4))
+    p.green_init(0.03, 9.0)
+    addr = np.arange(64)
+    p.set_target(addr, np.full(192, 3.0 + 0j), np.full(192, 1 / 3.0 + 0j))
+    B = torch.zeros((192, 4097), dtype=torch.complex128, device="cuda")
+    with pytest.raises(ValueError, match="4096"):
+        p.solve(B)
+
+
+@pytest.mark.parametrize("solver", ["cocg_rq", "bicgstab"])
 @pytest.mark.parametrize("L", [6, 24, 37, 64, 70])
 def test_solver_iterates_and_observables_match_oracle(solver, L):
     """Block solves against the oracle; L = 6, 24 and 70 put the Gram and
@@ -506,6 +536,35 @@ def test_cocg_nearly_dependent_orientations():
         one[6] = eu[l:l + 1]
         d1 = _gpu_dd(tuple(one), "bicgstab", 1e-10, 80)
         assert rel(dd.X[:, l], d1.X[:, 0]) < 1e-7
+
+
+@pytest.mark.parametrize("la_path", [0, 1])
+def test_repeated_orientations_as_the_reference(la_path):
+    """Four orientations given twice make the right-hand-side block exactly
+    rank deficient.  The reference's block-COCG-rq proceeds (numpy's
+    Householder QR gives a zero row in R) and solves both copies alike; its
+    block-BiCGSTAB raises LinAlgError (singular R0^H V).  Here even shifted
+    Cholesky QR breaks down on such a block, and the Householder QR on the
+    host takes over, on either L x L algebra path."""
+    say(f"repeated orientations la_path={la_path}")
+    prob = list(_problem(L=16))
+    eu = np.concatenate([prob[6], prob[6][:4]])
+    prob[6] = eu
+    prob = tuple(prob)
+    ref = _oracle_solve(prob, "cocg", 1e-8, 80)
+    from bdda_amd import _lib
+    lib = _lib.load()
+    try:
+        _lib.check(lib.bdda_set_la_path(la_path))
+        dd = _gpu_dd(prob, "cocg_rq", 1e-8, 80)
+        with pytest.raises(np.linalg.LinAlgError):
+            _gpu_dd(prob, "bicgstab", 1e-8, 80)
+    finally:
+        lib.bdda_set_la_path(0)
+    assert dd.converge and dd.true_residual < 1e-7
+    assert abs(dd.iter_fin - ref["iter_fin"]) <= 1 and rel(dd.X, ref["X"]) < 1e-6
+    assert rel(dd.X[:, 16:], dd.X[:, :4]) < 1e-8
+    np.testing.assert_allclose(dd.compute_C_ext(), ref["C_ext"], rtol=1e-6)
 
 
 def test_nonconvergence_is_reported_not_raised():
