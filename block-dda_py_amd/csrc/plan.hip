@@ -830,10 +830,11 @@ static int solve_cocg(Ctx& c, const V* Bj, V* X, bool x_zero, double bnorm, doub
     if (!la::lu_factor(LU, piv, L)) return singular("block-COCG-rq (Q^T Q)");
     beta = rhs;
     la::lu_solve(LU, piv, L, beta, L);
-    // Sn = Qn + S beta ; Q <- Qn
+    // Sn = Qn + S beta ; Q <- Qn by exchanging the Q and AS buffers (the
+    // old Q is not read again; the next product overwrites AS)
     if ((rc = update<V>(c, Sn, Qn, 1.0, S, nullptr, 0.0, beta, 2, nullptr, 0.0))) return rc;
     std::swap(S, Sn);
-    BDDA_HIP(hipMemcpyAsync(Q, Qn, bytes, hipMemcpyDeviceToDevice, c.s));
+    std::swap(Q, AS);
     QtQ.swap(QntQn);
   }
   *it_out = it;
