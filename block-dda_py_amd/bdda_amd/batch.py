@@ -65,6 +65,7 @@ def solve_batch(dds, nstreams=8):
     seconds = time.time() - t0
     for i, dd in enumerate(dds):
         nh = int(flags[i]) >> 8
+        dd.solve_info = {"refine_cycles": (int(flags[i]) >> 2) & 63, "fallback_complex128": bool(flags[i] & 2)}
         dd._store_solution(Xs[i], int(it[i]), float(err[i]), float(tr[i]), hist[i, :nh].tolist(),
                            bool(flags[i] & 1), seconds)
     return seconds

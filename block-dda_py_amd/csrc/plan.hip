@@ -857,9 +857,7 @@ static int solve_cocg(Ctx& c, const V* Bj, V* X, bool x_zero, double bnorm, doub
 // host algebra stays the default and this path is selected with
 // bdda_set_la_path(1) (one wait per iteration; a host with a slower CPU or
 // a caller that wants no host work inside the iteration).
-// 0: host algebra, 1: device algebra when L <= kDlaMax, 2: the same with
-// each iteration's launches replayed from a HIP graph
-std::atomic<int> g_la_path{0};
+std::atomic<int> g_la_path{0};  // 0: host algebra, 1: device algebra when L <= kDlaMax
 
 enum DlaSlot {
   D_QTQ, D_STAS, D_ALPHA, D_NALPHA, D_AX, D_G, D_R1, D_R1I, D_R2, D_R2I, D_R3, D_R3I, D_TAU, D_TAU2, D_XI, D_XIP,
@@ -879,55 +877,7 @@ struct Dla {
   DlaMisc* host;
 };
 
-static bool use_dla(const Ctx& c) { return g_la_path.load() >= 1 && c.L <= kDlaMax; }
-static bool use_graphs(const Ctx& c) { return g_la_path.load() == 2 && !c.p->profiling; }
-
-// Iteration graphs of the device-algebra solvers.  With the L x L algebra
-// on the device, everything between two convergence waits is a fixed
-// sequence of launches and device copies on one stream (no memset, no host
-// arithmetic, no allocation once the first iteration has sized every
-// buffer), whose pointers alternate between two sets from one iteration to
-// the next.  Each set's sequence is captured once (thread-local capture, so
-// other threads' streams are unaffected) and then replayed with one
-// hipGraphLaunch per iteration.  A capture or instantiation the runtime
-// refuses leaves the solve on direct launches.
-struct IterGraphs {
-  hipGraphExec_t exec[2] = {nullptr, nullptr};
-  bool off = false;
-  IterGraphs() = default;
-  IterGraphs(const IterGraphs&) = delete;
-  IterGraphs& operator=(const IterGraphs&) = delete;
-  ~IterGraphs() {
-    for (hipGraphExec_t e : exec)
-      if (e) (void)hipGraphExecDestroy(e);
-  }
-  // Issue body()'s work for pointer set `parity` on c.s.
-  template <typename F>
-  int run(Ctx& c, int parity, F&& body) {
-    if (off) return body();
-    hipGraphExec_t& e = exec[parity];
-    if (!e) {
-      if (hipStreamBeginCapture(c.s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-        (void)hipGetLastError();
-        off = true;
-        return body();
-      }
-      const int rc = body();
-      hipGraph_t g = nullptr;
-      const hipError_t ce = hipStreamEndCapture(c.s, &g);
-      const bool ok = rc == 0 && ce == hipSuccess && g && hipGraphInstantiate(&e, g, nullptr, nullptr, 0) == hipSuccess;
-      if (g) (void)hipGraphDestroy(g);
-      if (!ok) {
-        (void)hipGetLastError();
-        e = nullptr;
-        off = true;
-        return body();  // nothing of the refused capture ran
-      }
-    }
-    BDDA_HIP(hipGraphLaunch(e, c.s));
-    return 0;
-  }
-};
+static bool use_dla(const Ctx& c) { return g_la_path.load() == 1 && c.L <= kDlaMax; }
 
 static int dla_setup(Ctx& c, Dla& d) {
   bdda_plan* p = c.p;
@@ -1072,59 +1022,26 @@ static int solve_cocg_dev(Ctx& c, const double2* Bj, double2* X, bool x_zero, do
   BDDA_HIP(hipMemcpyAsync(d.slot[D_XI], d.slot[D_TAU], lbytes, hipMemcpyDeviceToDevice, c.s));
   BDDA_HIP(hipMemcpyAsync(S, Q, bytes, hipMemcpyDeviceToDevice, c.s));
   if ((rc = gram_dev(c, Q, Q, false, d.slot[D_QTQ]))) return rc;
-  // S of even iterations in ws[1], of odd ones in ws[5]; [Qn, tau] = qr(W)
-  // goes to AS, free until the next product
-  double2* const Sbuf[2] = {S, Sn};
-  double2* const Qn = AS;
-  // beta of iteration k - 1: beta = (Q^T Q)^-1 tau^T (Qn^T Qn),
-  // S_k = Qn + S_(k-1) beta, Q <- Qn
-  auto post = [&](int k) -> int {
-    if (int e = gram_dev(c, Qn, Qn, false, d.slot[D_QNQN])) return e;
-    dla_matmul(d.slot[D_TAU], true, d.slot[D_QNQN], d.slot[D_RHS], c.L, c.s);
-    dla_lu_solve(d.slot[D_QTQ], d.slot[D_RHS], c.L, c.L, false, d.slot[D_BETA], nullptr, nullptr, nullptr,
-                 &d.misc->status, c.s);
-    if (int e = update_dev(c, Sbuf[k & 1], Qn, 1.0, Sbuf[(k - 1) & 1], nullptr, d.slot[D_BETA], nullptr, nullptr))
-      return e;
-    BDDA_HIP(hipMemcpyAsync(Q, Qn, bytes, hipMemcpyDeviceToDevice, c.s));
-    BDDA_HIP(hipMemcpyAsync(d.slot[D_QTQ], d.slot[D_QNQN], lbytes, hipMemcpyDeviceToDevice, c.s));
-    return 0;
-  };
-  // iteration k up to its wait: alpha, X, W = Q - AS alpha, its QR, xi, err
-  auto pre = [&](int k) -> int {
-    double2* const Sk = Sbuf[k & 1];
-    if (int e = op<double2>(c, Sk, AS)) return e;
-    if (int e = gram_dev(c, Sk, AS, false, d.slot[D_STAS])) return e;
+  int it = 0;
+  double err = INFINITY;
+  for (int k = 0; k < itermax; ++k) {
+    bool stop = false;
+    if ((rc = conv32_check(c, X, Bj, AS, W, bnorm, err, tol, k, &stop))) return rc;
+    if (stop) break;
+    if ((rc = op<double2>(c, S, AS))) return rc;
+    if ((rc = gram_dev(c, S, AS, false, d.slot[D_STAS]))) return rc;
     dla_lu_solve(d.slot[D_STAS], d.slot[D_QTQ], c.L, c.L, false, d.slot[D_ALPHA], d.slot[D_NALPHA], nullptr,
                  nullptr, &d.misc->status, c.s);
     dla_matmul(d.slot[D_ALPHA], false, d.slot[D_XI], d.slot[D_AX], c.L, c.s);
     // X = X + S (alpha xi) ; W = Q - AS alpha
-    if (int e = update_dev(c, X, X, 1.0, Sk, nullptr, d.slot[D_AX], nullptr, nullptr)) return e;
-    if (int e = update_dev(c, W, Q, 1.0, AS, nullptr, d.slot[D_NALPHA], nullptr, nullptr)) return e;
-    // [Qn, tau] = qr(W) ; xi = tau xi
-    if (int e = cholqr2_dev(c, d, W, Q1, Qn)) return e;
+    if ((rc = update_dev(c, X, X, 1.0, S, nullptr, d.slot[D_AX], nullptr, nullptr))) return rc;
+    if ((rc = update_dev(c, W, Q, 1.0, AS, nullptr, d.slot[D_NALPHA], nullptr, nullptr))) return rc;
+    // [Qn, tau] = qr(W) into AS (free until the next product); xi = tau xi
+    double2* Qn = AS;
+    if ((rc = cholqr2_dev(c, d, W, Q1, Qn))) return rc;
     BDDA_HIP(hipMemcpyAsync(d.slot[D_XIP], d.slot[D_XI], lbytes, hipMemcpyDeviceToDevice, c.s));
     dla_matmul(d.slot[D_TAU], false, d.slot[D_XIP], d.slot[D_XI], c.L, c.s);
     dla_frob(d.slot[D_XI], c.L, bnorm, &d.misc->err, c.s);
-    return 0;
-  };
-  IterGraphs graphs;
-  const bool replay = use_graphs(c);
-  int it = 0;
-  double err = INFINITY;
-  for (int k = 0; k < itermax; ++k) {
-    const bool check = c.conv32 && c.check_every > 0 && k > 0 && k % c.check_every == 0;
-    if (k == 0 || check || !replay) {
-      if (k > 0 && (rc = post(k))) return rc;
-      bool stop = false;
-      if ((rc = conv32_check(c, X, Bj, AS, W, bnorm, err, tol, k, &stop))) return rc;
-      if (stop) break;
-      if ((rc = pre(k))) return rc;
-    } else if ((rc = graphs.run(c, k & 1, [&]() -> int {
-                 const int e = post(k);
-                 return e ? e : pre(k);
-               }))) {
-      return rc;
-    }
     if ((rc = dla_sync(c, d, &err, &st))) return rc;
     if ((st & DLA_SHIFTED) && !(st & (DLA_SINGULAR | DLA_NOT_PD))) {
       int st3;
@@ -1145,6 +1062,15 @@ static int solve_cocg_dev(Ctx& c, const double2* Bj, double2* X, bool x_zero, do
     if (c.hist) c.hist->push_back(err);
     it = k;
     if (err < tol) break;
+    // beta = (Q^T Q)^-1 tau^T (Qn^T Qn) ; Sn = Qn + S beta ; Q <- Qn
+    if ((rc = gram_dev(c, Qn, Qn, false, d.slot[D_QNQN]))) return rc;
+    dla_matmul(d.slot[D_TAU], true, d.slot[D_QNQN], d.slot[D_RHS], c.L, c.s);
+    dla_lu_solve(d.slot[D_QTQ], d.slot[D_RHS], c.L, c.L, false, d.slot[D_BETA], nullptr, nullptr, nullptr,
+                 &d.misc->status, c.s);
+    if ((rc = update_dev(c, Sn, Qn, 1.0, S, nullptr, d.slot[D_BETA], nullptr, nullptr))) return rc;
+    std::swap(S, Sn);
+    BDDA_HIP(hipMemcpyAsync(Q, Qn, bytes, hipMemcpyDeviceToDevice, c.s));
+    BDDA_HIP(hipMemcpyAsync(d.slot[D_QTQ], d.slot[D_QNQN], lbytes, hipMemcpyDeviceToDevice, c.s));
   }
   *it_out = it;
   *err_out = err;
@@ -1171,60 +1097,38 @@ static int solve_bicgstab_dev(Ctx& c, const double2* Bj, double2* X, bool x_zero
   BDDA_HIP(hipMemcpyAsync(P, R, bytes, hipMemcpyDeviceToDevice, c.s));
   BDDA_HIP(hipMemcpyAsync(R0, R, bytes, hipMemcpyDeviceToDevice, c.s));
   const double2* sc = d.misc->sc;  // -xi, xi
-  // P of even iterations in ws[1], of odd ones in ws[6]
-  double2* const Pbuf[2] = {P, Pn};
-  // beta of iteration k - 1: beta = (R0^H V)^-1 (-R0^H Z),
-  // P_k = R + (P_(k-1) - xi V) beta
-  auto post = [&](int k) -> int {
-    dla_lu_resolve(d.slot[D_LU], d.misc->piv, d.slot[D_RZ], c.L, c.L, true, d.slot[D_BETA], &d.misc->status, c.s);
-    return update_dev(c, Pbuf[k & 1], R, 1.0, Pbuf[(k - 1) & 1], Vb, d.slot[D_BETA], nullptr, sc);
-  };
-  // iteration k up to its wait: alpha, T, Z, xi, X, R, R0^H Z and err
-  auto pre = [&](int k) -> int {
-    double2* const Pk = Pbuf[k & 1];
-    if (int e = op<double2>(c, Pk, Vb)) return e;
-    if (int e = gram_dev(c, R0, Vb, true, d.slot[D_RV])) return e;
-    if (int e = gram_dev(c, R0, R, true, d.slot[D_RR])) return e;
+  int it = 0, st = 0;
+  double err = INFINITY;
+  for (int k = 0; k < itermax; ++k) {
+    bool stop = false;
+    if ((rc = conv32_check(c, X, Bj, T, Z, bnorm, err, tol, k, &stop))) return rc;
+    if (stop) break;
+    if ((rc = op<double2>(c, P, Vb))) return rc;
+    if ((rc = gram_dev(c, R0, Vb, true, d.slot[D_RV])) || (rc = gram_dev(c, R0, R, true, d.slot[D_RR]))) return rc;
     dla_lu_solve(d.slot[D_RV], d.slot[D_RR], c.L, c.L, false, d.slot[D_ALPHA], d.slot[D_NALPHA], d.slot[D_LU],
                  d.misc->piv, &d.misc->status, c.s);
     // T = R - V alpha
-    if (int e = update_dev(c, T, R, 1.0, Vb, nullptr, d.slot[D_NALPHA], nullptr, nullptr)) return e;
-    if (int e = op<double2>(c, T, Z)) return e;
+    if ((rc = update_dev(c, T, R, 1.0, Vb, nullptr, d.slot[D_NALPHA], nullptr, nullptr))) return rc;
+    if ((rc = op<double2>(c, T, Z))) return rc;
     // xi = tr(Z^H T) / tr(Z^H Z) from one pass over T and Z
     launch_dot<double2>(T, Z, n, c.p->dots_d.as<double2>(), c.p->dots_n.as<double>(), dot_parts(), c.s);
     BDDA_HIP(hipGetLastError());
     dla_xi(c.p->dots_d.as<double2>(), c.p->dots_n.as<double>(), dot_parts(), &d.misc->status, d.misc->sc, c.s);
     // X = X + P alpha + xi T ; R = T - xi Z with its norm ; R0^H Z
-    if (int e = update_dev(c, X, X, 1.0, Pk, nullptr, d.slot[D_ALPHA], T, sc)) return e;
+    if ((rc = update_dev(c, X, X, 1.0, P, nullptr, d.slot[D_ALPHA], T, sc))) return rc;
     launch_axpy_norm<double2>(R, T, make_double2(0.0, 0.0), Z, n, c.p->dots_n.as<double>(), dot_parts(), c.s, sc);
     BDDA_HIP(hipGetLastError());
-    if (int e = gram_dev(c, R0, Z, true, d.slot[D_RZ])) return e;
+    if ((rc = gram_dev(c, R0, Z, true, d.slot[D_RZ]))) return rc;
     dla_norm_err(c.p->dots_n.as<double>(), dot_parts(), bnorm, &d.misc->err, c.s);
-    return 0;
-  };
-  IterGraphs graphs;
-  const bool replay = use_graphs(c);
-  int it = 0, st = 0;
-  double err = INFINITY;
-  for (int k = 0; k < itermax; ++k) {
-    const bool check = c.conv32 && c.check_every > 0 && k > 0 && k % c.check_every == 0;
-    if (k == 0 || check || !replay) {
-      if (k > 0 && (rc = post(k))) return rc;
-      bool stop = false;
-      if ((rc = conv32_check(c, X, Bj, T, Z, bnorm, err, tol, k, &stop))) return rc;
-      if (stop) break;
-      if ((rc = pre(k))) return rc;
-    } else if ((rc = graphs.run(c, k & 1, [&]() -> int {
-                 const int e = post(k);
-                 return e ? e : pre(k);
-               }))) {
-      return rc;
-    }
     if ((rc = dla_sync(c, d, &err, &st))) return rc;
     if ((rc = dla_status_error(st, "block-BiCGSTAB (R0^H V)"))) return rc;
     if (c.hist) c.hist->push_back(err);
     it = k;
     if (err < tol) break;
+    // beta = (R0^H V)^-1 (-R0^H Z) ; Pn = R + (P - xi V) beta
+    dla_lu_resolve(d.slot[D_LU], d.misc->piv, d.slot[D_RZ], c.L, c.L, true, d.slot[D_BETA], &d.misc->status, c.s);
+    if ((rc = update_dev(c, Pn, R, 1.0, P, Vb, d.slot[D_BETA], nullptr, sc))) return rc;
+    std::swap(P, Pn);
   }
   *it_out = it;
   *err_out = err;
@@ -1295,9 +1199,8 @@ const char* bdda_last_error(void) { return g_err.c_str(); }
 int bdda_version(void) { return 100; }
 
 int bdda_set_la_path(int path) {
-  if (path < 0 || path > 2) {
-    set_error("bdda_set_la_path: path must be 0 (host algebra), 1 (device algebra for L <= 64) or 2 (device "
-              "algebra with iterations replayed from HIP graphs)");
+  if (path < 0 || path > 1) {
+    set_error("bdda_set_la_path: path must be 0 (host algebra) or 1 (device algebra for L <= 64)");
     return BDDA_E_ARG;
   }
   g_la_path = path;

@@ -59,10 +59,7 @@ int bdda_set_fft_path(int path);
  * partial pivoting, Cholesky, triangular inverse; 3-4 waits per iteration);
  * 1 on the device for L <= 64 (the same routines in single-workgroup
  * kernels; one wait per iteration, for the convergence test), on the host
- * for larger L; 2 as 1, with every iteration after the first replayed from
- * a HIP graph captured on the solve's stream (one launch and one wait per
- * iteration; results equal path 1's bit for bit; no graphs while
- * bdda_profile_enable is on).  Any other value: BDDA_E_ARG. */
+ * for larger L.  On the MI355X box both take about the same time. */
 int bdda_set_la_path(int path);
 
 /* Plan over an nx x ny x nz cuboid lattice: chooses the convolution FFT
