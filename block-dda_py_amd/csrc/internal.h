@@ -120,6 +120,15 @@ template <typename V>
 void launch_update(V* Y, const V* X, double2 a, const V* W1, const V* W2, double2 w2c,
                    const double2* C, const V* E, double2 e, long long rows, int L, hipStream_t s,
                    const double2* sc = nullptr);  // sc (device, nullable): w2c = sc[0], e = sc[1]
+// The same update for complex128 blocks with L <= 32 on the float64 matrix
+// cores; gram = 1 (Y^H Y) or 2 (Y^T Y) also writes the Gram matrix of the new
+// block Y into out (L x L, device), through update_gram_parts(rows) partial
+// TL x TL tiles in `partial`; gram = 0 is the update alone.
+void launch_update_gram(double2* Y, const double2* X, double2 a, const double2* W1, const double2* W2, double2 w2c,
+                        const double2* C, const double2* E, double2 e, long long rows, int L, int gram,
+                        double2* partial, double2* out, hipStream_t s, const double2* sc = nullptr);
+int update_gram_parts(long long rows);
+constexpr int kUpdateGramMaxL = 32;
 // Y = X + c*Z, and partial |Y|^2 sums (nullable).
 template <typename V>
 void launch_axpy_norm(V* Y, const V* X, double2 c, const V* Z, long long n, double* partial,

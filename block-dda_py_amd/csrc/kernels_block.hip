@@ -462,6 +462,190 @@ __global__ void __launch_bounds__(256) k_update_mfma(double2* Y, const double2* 
   }
 }
 
+// The update for complex128 blocks with L <= 16 NT (NT = 1 or 2) on the
+// float64 matrix cores, optionally with the Gram matrix op(Y)^T Y of the
+// block it writes (GRAM 1: Y^H Y, 2: Y^T Y), so that the Gram needs no second
+// pass over Y.  Wave w of a block takes rows 16 w .. +16 of each 64-row slab
+// against all 16 NT columns, as in k_update_mfma.  Entry r of tile t of lane
+// l is Y[row kr + 4 r][column 16 t + cc] (kr = l / 16, cc = l % 16), which is
+// exactly the operand layout of v_mfma_f64_16x16x4f64 over the four rows
+// kr + 4 r (A[i][k] = op(Y[k][16 J + i]) from lane i + 16 k, B[k][j] =
+// Y[k][16 T + j] from the same lane): the Gram tiles (J <= T) accumulate from
+// the lanes' own registers.  Rows beyond `rows` hold zero.  At the end the
+// four waves' tiles are summed through LDS in the fixed order
+// ((w3 + w2) + w1) + w0 and written as the block's partial (TL x TL, the
+// lower tiles mirrored: conjugated for GRAM 1), which k_gram_reduce adds up
+// in a fixed order: repeated calls agree bit for bit.
+template <int NT, int GRAM>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT == 1 ? 4 : (GRAM ? 2 : 3)))) k_update_gram(double2* Y, const double2* X, double2 a, const double2* W1,
+                                                     const double2* W2, double2 w2c,
+                                                     const double2* __restrict__ C, const double2* E, double2 e,
+                                                     long long rows, int L, const double2* sc,
+                                                     double2* __restrict__ partial) {
+  constexpr int TL = 16 * NT, KS = TL / 4, NP = NT * (NT + 1) / 2;
+  constexpr bool CONJ = GRAM == 1;
+  __shared__ double2 Cs[TL][TL];
+  if (sc) {
+    w2c = sc[0];
+    e = sc[1];
+  }
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int kr = lane >> 4, cc = lane & 15;
+  for (int idx = tid; idx < TL * TL; idx += 256) {
+    const int j = idx / TL, l = idx % TL;
+    Cs[j][l] = (j < L && l < L) ? C[(long long)j * L + l] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  gram_d4 gr[NP], gi[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    gr[p] = gram_d4{0.0, 0.0, 0.0, 0.0};
+    gi[p] = gram_d4{0.0, 0.0, 0.0, 0.0};
+  }
+  for (long long r0 = blockIdx.x * 64LL; r0 < rows; r0 += 64LL * gridDim.x) {
+    const long long row = r0 + 16 * w + cc;  // this lane's W row
+    gram_d4 yr[NT], yi[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      yr[t] = gram_d4{0.0, 0.0, 0.0, 0.0};
+      yi[t] = gram_d4{0.0, 0.0, 0.0, 0.0};
+    }
+#pragma unroll
+    for (int st = 0; st < KS; ++st) {
+      const int j = 4 * st + kr;
+      double2 v = make_double2(0.0, 0.0);
+      if (row < rows && j < L) {
+        v = W1[row * L + j];
+        if (W2) {
+          const double2 u = W2[row * L + j];
+          v.x += w2c.x * u.x - w2c.y * u.y;
+          v.y += w2c.x * u.y + w2c.y * u.x;
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const double2 ct = Cs[j][16 * t + cc];
+        yr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(v.x, ct.x, yr[t], 0, 0, 0);
+        yi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(v.x, ct.y, yi[t], 0, 0, 0);
+        yr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-v.y, ct.y, yr[t], 0, 0, 0);
+        yi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(v.y, ct.x, yi[t], 0, 0, 0);
+      }
+    }
+    double2 yv[NT][4];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long long orow = r0 + 16 * w + kr + 4 * r;
+        const int l = 16 * t + cc;
+        double2 y = make_double2(0.0, 0.0);
+        if (orow < rows && l < L) {
+          const long long o = orow * L + l;
+          y = make_double2(yr[t][r], yi[t][r]);
+          if (X) {
+            const double2 x = X[o];
+            y.x += a.x * x.x - a.y * x.y;
+            y.y += a.x * x.y + a.y * x.x;
+          }
+          if (E) {
+            const double2 ev = E[o];
+            y.x += e.x * ev.x - e.y * ev.y;
+            y.y += e.x * ev.y + e.y * ev.x;
+          }
+          Y[o] = y;
+        }
+        yv[t][r] = y;
+      }
+    if constexpr (GRAM != 0) {
+      // conj:  Re += ar br + ai bi, Im += ar bi - ai br
+      // plain: Re += ar br - ai bi, Im += ar bi + ai br
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int p = 0;
+#pragma unroll
+        for (int J = 0; J < NT; ++J)
+#pragma unroll
+          for (int T = J; T < NT; ++T, ++p) {
+            const double2 av = yv[J][r], bv = yv[T][r];
+            gr[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.x, bv.x, gr[p], 0, 0, 0);
+            gi[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.x, bv.y, gi[p], 0, 0, 0);
+            gr[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(CONJ ? av.y : -av.y, bv.y, gr[p], 0, 0, 0);
+            gi[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(CONJ ? -av.y : av.y, bv.x, gi[p], 0, 0, 0);
+          }
+      }
+    }
+  }
+  if constexpr (GRAM != 0) {
+    // ((w3 + w2) + w1) + w0 through Cs (free now), then the block's partial
+    __syncthreads();
+    for (int src = 3; src >= 0; --src) {
+      if (w == src) {
+        int p = 0;
+#pragma unroll
+        for (int J = 0; J < NT; ++J)
+#pragma unroll
+          for (int T = J; T < NT; ++T, ++p)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int j = 16 * J + kr + 4 * r, l = 16 * T + cc;
+              double2 g = make_double2(gr[p][r], gi[p][r]);
+              if (src < 3) g = zadd(Cs[j][l], g);
+              if (src > 0) {
+                Cs[j][l] = g;
+              } else {
+                double2* out = partial + (long long)blockIdx.x * (TL * TL);
+                out[j * TL + l] = g;
+                if (J < T) out[l * TL + j] = CONJ ? make_double2(g.x, -g.y) : g;
+              }
+            }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Blocks of the matrix-core update (and Gram) kernel for complex128 blocks
+// with L <= 32: one per 64-row slab, at most kUpdateGramBlocks.
+constexpr int kUpdateGramBlocks = 1024;
+int update_gram_parts(long long rows) { return (int)std::min<long long>((rows + 63) / 64, kUpdateGramBlocks); }
+
+template <int NT>
+static void launch_update_gram_nt(double2* Y, const double2* X, double2 a, const double2* W1, const double2* W2,
+                                  double2 w2c, const double2* C, const double2* E, double2 e, long long rows, int L,
+                                  int gram, double2* partial, double2* out, hipStream_t s, const double2* sc) {
+  const int g = update_gram_parts(rows);
+  switch (gram) {
+    case 1:
+      hipLaunchKernelGGL((k_update_gram<NT, 1>), dim3(g), dim3(256), 0, s, Y, X, a, W1, W2, w2c, C, E, e, rows, L,
+                         sc, partial);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_update_gram<NT, 2>), dim3(g), dim3(256), 0, s, Y, X, a, W1, W2, w2c, C, E, e, rows, L,
+                         sc, partial);
+      break;
+    default:
+      hipLaunchKernelGGL((k_update_gram<NT, 0>), dim3(g), dim3(256), 0, s, Y, X, a, W1, W2, w2c, C, E, e, rows, L,
+                         sc, partial);
+      return;
+  }
+  // reduction block shape: about 32 parts per thread's chain
+  constexpr int TL = 16 * NT;
+  int G = 1;
+  while (G < 256 && g / G > 32) G *= 2;
+  const int E_ = 256 / G;
+  hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)((TL * TL + E_ - 1) / E_)), dim3(256), 0, s, partial, g, 1, TL,
+                     L, E_, out);
+}
+
+void launch_update_gram(double2* Y, const double2* X, double2 a, const double2* W1, const double2* W2, double2 w2c,
+                        const double2* C, const double2* E, double2 e, long long rows, int L, int gram,
+                        double2* partial, double2* out, hipStream_t s, const double2* sc) {
+  if (L <= 16)
+    launch_update_gram_nt<1>(Y, X, a, W1, W2, w2c, C, E, e, rows, L, gram, partial, out, s, sc);
+  else
+    launch_update_gram_nt<2>(Y, X, a, W1, W2, w2c, C, E, e, rows, L, gram, partial, out, s, sc);
+}
+
 template <typename V, int TL>
 static void launch_update_tile(V* Y, const V* X, double2 a, const V* W1, const V* W2, double2 w2c,
                                const double2* C, const V* E, double2 e, long long rows, int L, hipStream_t s,
@@ -485,6 +669,12 @@ static void launch_update_tile(V* Y, const V* X, double2 a, const V* W1, const V
 template <typename V>
 void launch_update(V* Y, const V* X, double2 a, const V* W1, const V* W2, double2 w2c, const double2* C,
                    const V* E, double2 e, long long rows, int L, hipStream_t s, const double2* sc) {
+  if constexpr (std::is_same<V, double2>::value) {
+    if (L <= kUpdateGramMaxL) {  // the matrix-core kernel without its Gram
+      launch_update_gram(Y, X, a, W1, W2, w2c, C, E, e, rows, L, 0, nullptr, nullptr, s, sc);
+      return;
+    }
+  }
   switch (block_tile(L)) {
     case 16: launch_update_tile<V, 16>(Y, X, a, W1, W2, w2c, C, E, e, rows, L, s, sc); break;
     case 32: launch_update_tile<V, 32>(Y, X, a, W1, W2, w2c, C, E, e, rows, L, s, sc); break;

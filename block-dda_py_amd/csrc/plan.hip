@@ -571,7 +571,33 @@ static int update(Ctx& c, V* Y, const V* X, zc a, const V* W1, const V* W2, zc w
   return 0;
 }
 
+// The update Y = a X + W1 C on the matrix cores with the Gram matrix of Y
+// (Y^H Y when conj, else Y^T Y) into pinned host slot `gslot` without
+// waiting (complex128 blocks, L <= kUpdateGramMaxL): the Gram needs no second
+// pass over Y.
+static int update_gram_launch(Ctx& c, double2* Y, const double2* X, zc a, const double2* W1,
+                              const std::vector<zc>& C, int cslot, bool conj, int gslot) {
+  bdda_plan* p = c.p;
+  const int L = c.L;
+  const size_t one = (size_t)L * L * sizeof(double2);
+  const double2* dC = nullptr;
+  int rc = coef_upload(c, C, cslot, &dC);
+  if (!rc) rc = p->gram_part.ensure((size_t)update_gram_parts(c.rows) * 32 * 32 * sizeof(double2));
+  if (!rc) rc = p->gram_out.ensure(kGramSlots * one);
+  if (!rc) rc = p->h_gram.ensure(kGramSlots * one);
+  if (rc) return rc;
+  double2* dout = p->gram_out.as<double2>() + (size_t)gslot * L * L;
+  launch_update_gram(Y, X, make_double2(a.real(), a.imag()), W1, nullptr, make_double2(0.0, 0.0), dC, nullptr,
+                     make_double2(0.0, 0.0), c.rows, L, conj ? 1 : 2, p->gram_part.as<double2>(), dout, c.s);
+  BDDA_HIP(hipGetLastError());
+  BDDA_HIP(hipMemcpyAsync(p->h_gram.as<char>() + gslot * one, dout, one, hipMemcpyDeviceToHost, c.s));
+  return 0;
+}
+
 template <typename V> static constexpr bool is_f32() { return sizeof(V) == sizeof(float2); }
+template <typename V> static bool fused_gram(const Ctx& c) {
+  return std::is_same<V, double2>::value && c.L <= kUpdateGramMaxL;
+}
 
 // Mixed mode, at iteration k of a complex64-convolution cycle: keep going
 // only while the recursive residual err is on a log-linear course to tol by
@@ -758,7 +784,37 @@ static int cholqr2(Ctx& c, const V* W, V* Q1, V* Qn, std::vector<zc>& tau) {
   return 0;
 }
 
+// Cholesky-QR2 of W given its Gram matrix G1 = W^H W (from the update that
+// wrote W), for complex128 blocks with L <= kUpdateGramMaxL: the two
+// passes' updates Q1 = W R1^-1 and Qn = Q1 R2^-1 produce the Gram matrices
+// of their outputs, Q1^H Q1 for the second pass and Qn^T Qn (the next beta's)
+// into host slot 1, launched but not waited for (*qnqn_pending).  When a
+// Cholesky factorisation fails, cholqr2 (shifted CholeskyQR3, then
+// Householder) takes over from W and Qn^T Qn is left to the caller.
+static int cholqr2_fused(Ctx& c, const double2* W, const std::vector<zc>& G1, double2* Q1, double2* Qn,
+                         std::vector<zc>& tau, bool* qnqn_pending) {
+  const int L = c.L;
+  std::vector<zc> R1, R2, Ri, G2;
+  *qnqn_pending = false;
+  int rc;
+  if (la::cholesky_upper(G1, R1, L)) {
+    la::upper_inverse(R1, Ri, L);
+    if ((rc = update_gram_launch(c, Q1, nullptr, 0.0, W, Ri, 3, true, 0)) || (rc = host_wait(c))) return rc;
+    gram_unpack(c, 0, G2);
+    if (la::cholesky_upper(G2, R2, L)) {
+      la::upper_inverse(R2, Ri, L);
+      if ((rc = update_gram_launch(c, Qn, nullptr, 0.0, Q1, Ri, 4, false, 1))) return rc;
+      *qnqn_pending = true;
+      la::matmul(R2, false, R1, tau, L);
+      return 0;
+    }
+  }
+  return cholqr2<double2>(c, W, Q1, Qn, tau);
+}
+
 // Block-COCG-rq on the preconditioned system (bl_krylov.py:58-118).
+// Complex128 blocks with L <= kUpdateGramMaxL take the Gram matrices of the
+// block QR and of the next beta from the updates that write those blocks.
 template <typename V>
 static int solve_cocg(Ctx& c, const V* Bj, V* X, bool x_zero, double bnorm, double tol, int itermax, V** ws,
                       int* it_out, double* err_out) {
@@ -813,10 +869,21 @@ static int solve_cocg(Ctx& c, const V* Bj, V* X, bool x_zero, double bnorm, doub
     // W = Q - AS alpha
     negA.resize(alpha.size());
     for (size_t i = 0; i < alpha.size(); ++i) negA[i] = -alpha[i];
-    if ((rc = update<V>(c, W, Q, 1.0, AS, nullptr, 0.0, negA, 1, nullptr, 0.0))) return rc;
     // [Qn, tau] = qr(W); Qn written into AS (free until the next product)
     V* Qn = AS;
-    if ((rc = cholqr2<V>(c, W, Q1, Qn, tau))) return rc;
+    bool qnqn_pending = false;
+    if constexpr (std::is_same<V, double2>::value) {
+      if (fused_gram<V>(c)) {
+        std::vector<zc> G1;
+        if ((rc = update_gram_launch(c, W, Q, 1.0, AS, negA, 1, true, 0)) || (rc = host_wait(c))) return rc;
+        gram_unpack(c, 0, G1);
+        if ((rc = cholqr2_fused(c, W, G1, Q1, Qn, tau, &qnqn_pending))) return rc;
+      }
+    }
+    if (!fused_gram<V>(c)) {
+      if ((rc = update<V>(c, W, Q, 1.0, AS, nullptr, 0.0, negA, 1, nullptr, 0.0))) return rc;
+      if ((rc = cholqr2<V>(c, W, Q1, Qn, tau))) return rc;
+    }
     std::vector<zc> xin;
     la::matmul(tau, false, xi, xin, L);
     xi.swap(xin);
@@ -824,7 +891,12 @@ static int solve_cocg(Ctx& c, const V* Bj, V* X, bool x_zero, double bnorm, doub
     if (c.hist) c.hist->push_back(err);
     it = k;
     if (err < tol) break;
-    if ((rc = gram_to_host(c, Qn, Qn, false, is_f32<V>(), QntQn))) return rc;
+    if (qnqn_pending) {
+      if ((rc = host_wait(c))) return rc;
+      gram_unpack(c, 1, QntQn);
+    } else if ((rc = gram_to_host(c, Qn, Qn, false, is_f32<V>(), QntQn))) {
+      return rc;
+    }
     la::matmul(tau, true, QntQn, rhs, L);  // tau^T (Qn^T Qn)
     LU = QtQ;
     if (!la::lu_factor(LU, piv, L)) return singular("block-COCG-rq (Q^T Q)");
@@ -921,17 +993,43 @@ static int update_dev(Ctx& c, double2* Y, const double2* X, zc a, const double2*
   return 0;
 }
 
+// Y = a X + W1 C (C on the device) on the matrix cores with the Gram matrix
+// of Y (Y^H Y when conj, else Y^T Y) into gram_out (device); complex128
+// blocks with L <= kUpdateGramMaxL.
+static int update_gram_dev(Ctx& c, double2* Y, const double2* X, zc a, const double2* W1, const double2* C,
+                           bool conj, double2* gram_out) {
+  if (int rc = c.p->gram_part.ensure((size_t)update_gram_parts(c.rows) * 32 * 32 * sizeof(double2))) return rc;
+  launch_update_gram(Y, X, make_double2(a.real(), a.imag()), W1, nullptr, make_double2(0.0, 0.0), C, nullptr,
+                     make_double2(0.0, 0.0), c.rows, c.L, conj ? 1 : 2, c.p->gram_part.as<double2>(), gram_out, c.s);
+  BDDA_HIP(hipGetLastError());
+  return 0;
+}
+
 // Cholesky QR2 of W into Qn with tau = R2 R1 (device): the first pass
 // shifts its Gram matrix if the plain factorisation fails (status
-// DLA_SHIFTED; the caller then runs dla_qr_third_pass).
-static int cholqr2_dev(Ctx& c, Dla& d, const double2* W, double2* Q1, double2* Qn) {
+// DLA_SHIFTED; the caller then runs dla_qr_third_pass).  With g_ready, W^H W
+// is already in slot D_G (from the update that wrote W).  Where the updates
+// carry their Gram matrices (fused_gram), Q1^H Q1 comes from the update that
+// writes Q1, and with qnqn the update that writes Qn leaves Qn^T Qn there.
+static int cholqr2_dev(Ctx& c, Dla& d, const double2* W, double2* Q1, double2* Qn, bool g_ready = false,
+                       double2* qnqn = nullptr) {
   int* st = &d.misc->status;
-  if (int rc = gram_dev(c, W, W, true, d.slot[D_G])) return rc;
+  const bool fused = fused_gram<double2>(c);
+  if (!g_ready)
+    if (int rc = gram_dev(c, W, W, true, d.slot[D_G])) return rc;
   dla_chol_inv(d.slot[D_G], c.L, (double)c.rows, true, d.slot[D_R1], d.slot[D_R1I], st, c.s);
-  if (int rc = update_dev(c, Q1, nullptr, 0.0, W, nullptr, d.slot[D_R1I], nullptr, nullptr)) return rc;
-  if (int rc = gram_dev(c, Q1, Q1, true, d.slot[D_G])) return rc;
+  if (fused) {
+    if (int rc = update_gram_dev(c, Q1, nullptr, 0.0, W, d.slot[D_R1I], true, d.slot[D_G])) return rc;
+  } else {
+    if (int rc = update_dev(c, Q1, nullptr, 0.0, W, nullptr, d.slot[D_R1I], nullptr, nullptr)) return rc;
+    if (int rc = gram_dev(c, Q1, Q1, true, d.slot[D_G])) return rc;
+  }
   dla_chol_inv(d.slot[D_G], c.L, (double)c.rows, false, d.slot[D_R2], d.slot[D_R2I], st, c.s);
-  if (int rc = update_dev(c, Qn, nullptr, 0.0, Q1, nullptr, d.slot[D_R2I], nullptr, nullptr)) return rc;
+  if (fused && qnqn) {
+    if (int rc = update_gram_dev(c, Qn, nullptr, 0.0, Q1, d.slot[D_R2I], false, qnqn)) return rc;
+  } else {
+    if (int rc = update_dev(c, Qn, nullptr, 0.0, Q1, nullptr, d.slot[D_R2I], nullptr, nullptr)) return rc;
+  }
   dla_matmul(d.slot[D_R2], false, d.slot[D_R1], d.slot[D_TAU], c.L, c.s);
   BDDA_HIP(hipGetLastError());
   return 0;
@@ -1035,10 +1133,18 @@ static int solve_cocg_dev(Ctx& c, const double2* Bj, double2* X, bool x_zero, do
     dla_matmul(d.slot[D_ALPHA], false, d.slot[D_XI], d.slot[D_AX], c.L, c.s);
     // X = X + S (alpha xi) ; W = Q - AS alpha
     if ((rc = update_dev(c, X, X, 1.0, S, nullptr, d.slot[D_AX], nullptr, nullptr))) return rc;
-    if ((rc = update_dev(c, W, Q, 1.0, AS, nullptr, d.slot[D_NALPHA], nullptr, nullptr))) return rc;
-    // [Qn, tau] = qr(W) into AS (free until the next product); xi = tau xi
+    // [Qn, tau] = qr(W) into AS (free until the next product); xi = tau xi.
+    // With fused Gram matrices W^H W comes with W and Qn^T Qn with Qn (the
+    // latter stands unless a fallback below rewrites Qn)
+    const bool fused = fused_gram<double2>(c);
+    bool qnqn_ready = fused;
+    if (fused) {
+      if ((rc = update_gram_dev(c, W, Q, 1.0, AS, d.slot[D_NALPHA], true, d.slot[D_G]))) return rc;
+    } else if ((rc = update_dev(c, W, Q, 1.0, AS, nullptr, d.slot[D_NALPHA], nullptr, nullptr))) {
+      return rc;
+    }
     double2* Qn = AS;
-    if ((rc = cholqr2_dev(c, d, W, Q1, Qn))) return rc;
+    if ((rc = cholqr2_dev(c, d, W, Q1, Qn, fused, fused ? d.slot[D_QNQN] : nullptr))) return rc;
     BDDA_HIP(hipMemcpyAsync(d.slot[D_XIP], d.slot[D_XI], lbytes, hipMemcpyDeviceToDevice, c.s));
     dla_matmul(d.slot[D_TAU], false, d.slot[D_XIP], d.slot[D_XI], c.L, c.s);
     dla_frob(d.slot[D_XI], c.L, bnorm, &d.misc->err, c.s);
@@ -1047,12 +1153,14 @@ static int solve_cocg_dev(Ctx& c, const double2* Bj, double2* X, bool x_zero, do
       int st3;
       BDDA_HIP(hipMemsetAsync(&d.misc->status, 0, sizeof(int), c.s));
       if ((rc = dla_qr_third_pass(c, d, Q1, Qn))) return rc;
+      qnqn_ready = false;
       dla_matmul(d.slot[D_TAU], false, d.slot[D_XIP], d.slot[D_XI], c.L, c.s);
       dla_frob(d.slot[D_XI], c.L, bnorm, &d.misc->err, c.s);
       if ((rc = dla_sync(c, d, &err, &st3))) return rc;
       st = (st & ~DLA_SHIFTED) | st3;
     }
     if ((st & DLA_NOT_PD) && !(st & DLA_SINGULAR)) {
+      qnqn_ready = false;
       if ((rc = householder_dev(c, d, W, Qn))) return rc;
       dla_matmul(d.slot[D_TAU], false, d.slot[D_XIP], d.slot[D_XI], c.L, c.s);
       dla_frob(d.slot[D_XI], c.L, bnorm, &d.misc->err, c.s);
@@ -1063,7 +1171,7 @@ static int solve_cocg_dev(Ctx& c, const double2* Bj, double2* X, bool x_zero, do
     it = k;
     if (err < tol) break;
     // beta = (Q^T Q)^-1 tau^T (Qn^T Qn) ; Sn = Qn + S beta ; Q <- Qn
-    if ((rc = gram_dev(c, Qn, Qn, false, d.slot[D_QNQN]))) return rc;
+    if (!qnqn_ready && (rc = gram_dev(c, Qn, Qn, false, d.slot[D_QNQN]))) return rc;
     dla_matmul(d.slot[D_TAU], true, d.slot[D_QNQN], d.slot[D_RHS], c.L, c.s);
     dla_lu_solve(d.slot[D_QTQ], d.slot[D_RHS], c.L, c.L, false, d.slot[D_BETA], nullptr, nullptr, nullptr,
                  &d.misc->status, c.s);

@@ -402,7 +402,11 @@ def test_device_and_host_la_paths_agree(solver, L, precision):
     np.testing.assert_allclose(d.err_history[0], h.err_history[0], rtol=1e-10 if L <= 8 else 1e-7)
     if L <= 8:
         assert d.iter_fin == h.iter_fin
-        np.testing.assert_allclose(d.err_history, h.err_history, rtol=1e-6)
+        # in mixed mode the residuals after the complex64 cycle carry that
+        # cycle's product rounding (about 1e-7 relative per product): the
+        # paths then agree to 1e-12 in the relative residual, not to 1e-6 of it
+        np.testing.assert_allclose(d.err_history, h.err_history, rtol=1e-6,
+                                   atol=1e-12 if precision == "mixed" else 0.0)
         assert rel(d.X, h.X) < 1e-9
     else:
         assert abs(d.iter_fin - h.iter_fin) <= 1
