@@ -114,7 +114,7 @@ void launch_green_expand(const double2* half, double2* full, const AxisPlan* ax,
 template <typename V>
 void launch_gram(const V* A, const V* B, long long rows, int L, bool conjA, double2* partial,
                  int nparts, double2* out, hipStream_t s);
-int gram_parts(int L);
+int gram_parts(int L, long long rows);
 // Y = a*X + sum_j (W1 + w2c*W2)[i][j] C[j][l] + e*E ;  X, W2, E nullable.
 template <typename V>
 void launch_update(V* Y, const V* X, double2 a, const V* W1, const V* W2, double2 w2c,

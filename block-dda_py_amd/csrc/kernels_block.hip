@@ -35,11 +35,15 @@ __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
 // partial[wg][tile](64x64) = sum_{rows of wg} op(A[i][jt*64+a]) * B[i][lt*64+b]
 // Row ranges (parts) per Gram: 512 up to L = 128; beyond, fewer, so that
 // the partial tiles (parts x ceil(L/64)^2 x 64 x 64) stay near 64 MB while
-// the grid keeps at least 512 blocks.
+// the grid keeps at least 512 blocks; and at least kGramMinRows rows per
+// part, so that on small blocks the partial tiles (and their reduction) do
+// not outweigh the rows they sum.
 constexpr int kGramParts = 512;
-int gram_parts(int L) {
+constexpr long long kGramMinRows = 128;
+int gram_parts(int L, long long rows) {
   const int nt = (L + 63) / 64;
-  return nt <= 2 ? kGramParts : std::max(8, 4 * kGramParts / (nt * nt));
+  const int most = nt <= 2 ? kGramParts : std::max(8, 4 * kGramParts / (nt * nt));
+  return (int)std::max(1LL, std::min<long long>(most, rows / kGramMinRows));
 }
 
 // A block owns a contiguous range of rows and one TL x TL output tile
@@ -47,16 +51,19 @@ int gram_parts(int L) {
 // multiply padding).  Its 256 threads form NG = 256 / (TL/4)^2 groups, each
 // with a 4 x 4 register tile per thread (rows ty + TL/4 i, columns
 // tx + TL/4 j: lanes walk consecutive LDS words, so reads are
-// conflict-free); group g takes the rows g, g + NG, ... of every 16-row slab
-// and writes its own partial tile.  The next slab is loaded into registers
-// while the current one is multiplied.
+// conflict-free); group g takes the rows g, g + NG, ... of every 16-row slab.
+// The next slab is loaded into registers while the current one is
+// multiplied.  At the end the groups' tiles are summed in group order
+// through LDS and the block writes one partial tile.
 template <typename V, bool CONJ, int TL>
 __global__ void __launch_bounds__(256) k_gram(const V* __restrict__ A, const V* __restrict__ B, long long rows,
                                               int L, double2* __restrict__ partial) {
   constexpr int TQ = TL / 4, TPG = TQ * TQ, NG = 256 / TPG;
   constexpr int PER = (16 * TL) / 256;  // slab entries per thread
-  __shared__ double2 As[16][TL];
-  __shared__ double2 Bs[16][TL];
+  static_assert(NG == 1 || TL * TL <= 32 * TL, "the group sum reuses the slab buffers");
+  __shared__ double2 sh[32 * TL];
+  double2 (*As)[TL] = reinterpret_cast<double2 (*)[TL]>(sh);
+  double2 (*Bs)[TL] = reinterpret_cast<double2 (*)[TL]>(sh + 16 * TL);
   const int tid = threadIdx.x, g = tid / TPG, t = tid % TPG, tx = t % TQ, ty = t / TQ;
   const int ntile = (L + TL - 1) / TL;
   const int jt = blockIdx.y / ntile, lt = blockIdx.y % ntile;
@@ -109,11 +116,28 @@ __global__ void __launch_bounds__(256) k_gram(const V* __restrict__ A, const V* 
         }
     }
   }
-  double2* out = partial + (((long long)blockIdx.x * NG + g) * gridDim.y + blockIdx.y) * (TL * TL);
+  double2* out = partial + ((long long)blockIdx.x * gridDim.y + blockIdx.y) * (TL * TL);
+  if constexpr (NG == 1) {
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) out[(ty + TQ * i) * TL + tx + TQ * j] = acc[i][j];
+      for (int j = 0; j < 4; ++j) out[(ty + TQ * i) * TL + tx + TQ * j] = acc[i][j];
+  } else {
+    __syncthreads();  // the last slab has been consumed
+    for (int q = 0; q < NG; ++q) {
+      if (g == q) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int idx = (ty + TQ * i) * TL + tx + TQ * j;
+            sh[idx] = q == 0 ? acc[i][j] : zadd(sh[idx], acc[i][j]);
+          }
+      }
+      __syncthreads();
+    }
+    for (int idx = tid; idx < TL * TL; idx += 256) out[idx] = sh[idx];
+  }
 }
 
 // The TL x TL tile (TL = 64 or 32) on the float64 matrix cores (complex128
@@ -225,6 +249,15 @@ __global__ void __launch_bounds__(256) k_gram_reduce(const double2* __restrict__
   }
 }
 
+// Entries per k_gram_reduce block for `parts` partial tiles: G = 256 / E
+// groups with chains of about 8 parts each, so that the reduction spreads
+// over many blocks.
+static int gram_reduce_width(int parts) {
+  int G = 1;
+  while (G < 256 && parts / G > 8) G *= 2;
+  return 256 / G;
+}
+
 // Output tile of the Gram and update kernels: the smallest of 16, 32, 64
 // that holds L (64 beyond).
 static int block_tile(int L) { return L <= 16 ? 16 : (L <= 32 ? 32 : 64); }
@@ -237,7 +270,6 @@ static void launch_gram_tile(const V* A, const V* B, long long rows, int L, bool
   // is bound by HBM reads, and the vector kernel's four row groups keep
   // more of them in flight).
   constexpr bool mfma = std::is_same<V, double2>::value && TL == 64;
-  constexpr int NG = mfma ? 1 : 256 / ((TL / 4) * (TL / 4));
   const int ntile = (L + TL - 1) / TL;
   dim3 grid(nparts, ntile * ntile);
   if constexpr (mfma) {
@@ -250,11 +282,8 @@ static void launch_gram_tile(const V* A, const V* B, long long rows, int L, bool
   } else {
     hipLaunchKernelGGL((k_gram<V, false, TL>), grid, dim3(256), 0, s, A, B, rows, L, partial);
   }
-  // reduction block shape: about 32 parts per thread's chain
-  const int parts = nparts * NG;
-  int G = 1;
-  while (G < 256 && parts / G > 32) G *= 2;
-  const int E = 256 / G;
+  const int parts = nparts;
+  const int E = gram_reduce_width(parts);
   const long long entries = (long long)ntile * ntile * TL * TL;
   hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)((entries + E - 1) / E)), dim3(256), 0, s, partial, parts, ntile,
                      TL, L, E, out);
@@ -477,7 +506,7 @@ __global__ void __launch_bounds__(256) k_update_mfma(double2* Y, const double2* 
 // lower tiles mirrored: conjugated for GRAM 1), which k_gram_reduce adds up
 // in a fixed order: repeated calls agree bit for bit.
 template <int NT, int GRAM>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT == 1 ? 4 : (GRAM ? 2 : 3)))) k_update_gram(double2* Y, const double2* X, double2 a, const double2* W1,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT == 1 ? 4 : 2))) k_update_gram(double2* Y, const double2* X, double2 a, const double2* W1,
                                                      const double2* W2, double2 w2c,
                                                      const double2* __restrict__ C, const double2* E, double2 e,
                                                      long long rows, int L, const double2* sc,
@@ -502,8 +531,65 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT == 
     gr[p] = gram_d4{0.0, 0.0, 0.0, 0.0};
     gi[p] = gram_d4{0.0, 0.0, 0.0, 0.0};
   }
+  const bool has_w2 = W2 != nullptr, has_x = X != nullptr, has_e = E != nullptr;
   for (long long r0 = blockIdx.x * 64LL; r0 < rows; r0 += 64LL * gridDim.x) {
+    // Every load of the slab is issued before the first use, from clamped
+    // (always valid) addresses; out-of-range entries are zeroed afterwards.
     const long long row = r0 + 16 * w + cc;  // this lane's W row
+    const long long rowc = row < rows ? row : rows - 1;
+    double2 v[KS];
+#pragma unroll
+    for (int st = 0; st < KS; ++st) {
+      const int j = 4 * st + kr;
+      v[st] = W1[rowc * L + (j < L ? j : L - 1)];
+    }
+    if (has_w2) {
+      double2 u[KS];
+#pragma unroll
+      for (int st = 0; st < KS; ++st) {
+        const int j = 4 * st + kr;
+        u[st] = W2[rowc * L + (j < L ? j : L - 1)];
+      }
+#pragma unroll
+      for (int st = 0; st < KS; ++st) {
+        v[st].x += w2c.x * u[st].x - w2c.y * u[st].y;
+        v[st].y += w2c.x * u[st].y + w2c.y * u[st].x;
+      }
+    }
+#pragma unroll
+    for (int st = 0; st < KS; ++st)
+      if (!(row < rows && 4 * st + kr < L)) v[st] = make_double2(0.0, 0.0);
+    // the epilogue's a X + e E at entries (row kr + 4 r, column 16 t + cc)
+    double2 xv[NT][4];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) xv[t][r] = make_double2(0.0, 0.0);
+    auto entry = [&](int t, int r) -> long long {
+      long long orow = r0 + 16 * w + kr + 4 * r;
+      orow = orow < rows ? orow : rows - 1;
+      const int l = 16 * t + cc;
+      return orow * L + (l < L ? l : L - 1);
+    };
+    if (has_x) {
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const double2 x = X[entry(t, r)];
+          xv[t][r] = make_double2(a.x * x.x - a.y * x.y, a.x * x.y + a.y * x.x);
+        }
+    }
+    if (has_e) {
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const double2 ee = E[entry(t, r)];
+          xv[t][r].x += e.x * ee.x - e.y * ee.y;
+          xv[t][r].y += e.x * ee.y + e.y * ee.x;
+        }
+    }
     gram_d4 yr[NT], yi[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
@@ -513,22 +599,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT == 
 #pragma unroll
     for (int st = 0; st < KS; ++st) {
       const int j = 4 * st + kr;
-      double2 v = make_double2(0.0, 0.0);
-      if (row < rows && j < L) {
-        v = W1[row * L + j];
-        if (W2) {
-          const double2 u = W2[row * L + j];
-          v.x += w2c.x * u.x - w2c.y * u.y;
-          v.y += w2c.x * u.y + w2c.y * u.x;
-        }
-      }
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
         const double2 ct = Cs[j][16 * t + cc];
-        yr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(v.x, ct.x, yr[t], 0, 0, 0);
-        yi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(v.x, ct.y, yi[t], 0, 0, 0);
-        yr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-v.y, ct.y, yr[t], 0, 0, 0);
-        yi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(v.y, ct.x, yi[t], 0, 0, 0);
+        yr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[st].x, ct.x, yr[t], 0, 0, 0);
+        yi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[st].x, ct.y, yi[t], 0, 0, 0);
+        yr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-v[st].y, ct.y, yr[t], 0, 0, 0);
+        yi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[st].y, ct.x, yi[t], 0, 0, 0);
       }
     }
     double2 yv[NT][4];
@@ -538,23 +615,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT == 
       for (int r = 0; r < 4; ++r) {
         const long long orow = r0 + 16 * w + kr + 4 * r;
         const int l = 16 * t + cc;
-        double2 y = make_double2(0.0, 0.0);
-        if (orow < rows && l < L) {
-          const long long o = orow * L + l;
-          y = make_double2(yr[t][r], yi[t][r]);
-          if (X) {
-            const double2 x = X[o];
-            y.x += a.x * x.x - a.y * x.y;
-            y.y += a.x * x.y + a.y * x.x;
-          }
-          if (E) {
-            const double2 ev = E[o];
-            y.x += e.x * ev.x - e.y * ev.y;
-            y.y += e.x * ev.y + e.y * ev.x;
-          }
-          Y[o] = y;
-        }
-        yv[t][r] = y;
+        const double2 y = make_double2(yr[t][r] + xv[t][r].x, yi[t][r] + xv[t][r].y);
+        const bool live = orow < rows && l < L;
+        if (live) Y[orow * L + l] = y;
+        yv[t][r] = live ? y : make_double2(0.0, 0.0);
       }
     if constexpr (GRAM != 0) {
       // conj:  Re += ar br + ai bi, Im += ar bi - ai br
@@ -605,8 +669,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT == 
 }
 
 // Blocks of the matrix-core update (and Gram) kernel for complex128 blocks
-// with L <= 32: one per 64-row slab, at most kUpdateGramBlocks.
-constexpr int kUpdateGramBlocks = 1024;
+// with L <= 32: one per 64-row slab, at most kUpdateGramBlocks (two waves
+// per SIMD on every CU for the 32-column kernels).
+constexpr int kUpdateGramBlocks = 512;
 int update_gram_parts(long long rows) { return (int)std::min<long long>((rows + 63) / 64, kUpdateGramBlocks); }
 
 template <int NT>
@@ -628,11 +693,8 @@ static void launch_update_gram_nt(double2* Y, const double2* X, double2 a, const
                          sc, partial);
       return;
   }
-  // reduction block shape: about 32 parts per thread's chain
   constexpr int TL = 16 * NT;
-  int G = 1;
-  while (G < 256 && g / G > 32) G *= 2;
-  const int E_ = 256 / G;
+  const int E_ = gram_reduce_width(g);
   hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)((TL * TL + E_ - 1) / E_)), dim3(256), 0, s, partial, g, 1, TL,
                      L, E_, out);
 }

@@ -445,7 +445,7 @@ static int gram_launch(Ctx& c, const void* A, const void* B, bool conj, bool f32
   bdda_plan* p = c.p;
   const int L = c.L;
   const int ntile = (L + 63) / 64;
-  const int np = gram_parts(L);
+  const int np = gram_parts(L, c.rows);
   const size_t one = (size_t)L * L * sizeof(double2);
   int rc = p->gram_part.ensure((size_t)np * ntile * ntile * 4096 * sizeof(double2));
   if (!rc) rc = p->gram_out.ensure(kGramSlots * one);
@@ -957,7 +957,7 @@ static int dla_setup(Ctx& c, Dla& d) {
   int rc = p->dla.ensure(kDlaSlots * one * sizeof(double2));
   if (!rc) rc = p->dla_misc.ensure(sizeof(DlaMisc));
   if (!rc) rc = p->h_sync.ensure(sizeof(DlaMisc));
-  if (!rc) rc = p->gram_part.ensure((size_t)gram_parts(c.L) * ((c.L + 63) / 64) * ((c.L + 63) / 64) * 4096 *
+  if (!rc) rc = p->gram_part.ensure((size_t)gram_parts(c.L, c.rows) * ((c.L + 63) / 64) * ((c.L + 63) / 64) * 4096 *
                                     sizeof(double2));
   if (!rc) rc = p->dots_d.ensure(dot_parts() * sizeof(double2));
   if (!rc) rc = p->dots_n.ensure(dot_parts() * sizeof(double));
@@ -970,7 +970,7 @@ static int dla_setup(Ctx& c, Dla& d) {
 }
 
 static int gram_dev(Ctx& c, const double2* A, const double2* B, bool conj, double2* out) {
-  launch_gram<double2>(A, B, c.rows, c.L, conj, c.p->gram_part.as<double2>(), gram_parts(c.L), out, c.s);
+  launch_gram<double2>(A, B, c.rows, c.L, conj, c.p->gram_part.as<double2>(), gram_parts(c.L, c.rows), out, c.s);
   BDDA_HIP(hipGetLastError());
   return 0;
 }
