@@ -277,7 +277,7 @@ def batch_leg(torch, nstreams=8):
     a warm-up)."""
     import numpy as np
     from bdda_amd import workloads
-    from bdda_amd.batch import solve_batch
+    from bdda_amd.batch import autotune, solve_batch
     wl = workloads.BATCH_32
     insts = workloads.batch_instances(wl)
     t0 = time.time()
@@ -317,6 +317,27 @@ def batch_leg(torch, nstreams=8):
             "max_true_residual": max(dd.true_residual for dd in dds),
             "iterations": {"min": min(its), "max": max(its), "mean": round(float(np.mean(its)), 2)},
         }
+        # the same batch with every plan's column tile autotuned (tuning is
+        # plan set-up, outside the timed region)
+        t0 = time.perf_counter()
+        tiles = autotune(dds)
+        torch.cuda.synchronize()
+        t_tune = time.perf_counter() - t0
+        X_bat = [dd.X_device.clone() for dd in dds]
+        solve_batch(dds, nstreams)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        solve_batch(dds, nstreams)
+        torch.cuda.synchronize()
+        t_tuned = time.perf_counter() - t0
+        out[prec]["autotuned"] = {
+            "value": round(n_or / t_tuned, 2), "batched_s": round(t_tuned, 4), "tuning_s": round(t_tune, 3),
+            "tiles": {str(t): tiles.count(t) for t in sorted(set(tiles))},
+            "max_rel_diff_vs_untuned": max(float((dd.X_device - x).abs().max() / x.abs().max())
+                                           for dd, x in zip(dds, X_bat)),
+            "all_converged": all(dd.converge for dd in dds)}
+        for dd in dds:
+            dd._plan.set_column_tile(0)
     out["value"] = out[wl["precision"]]["value"]
     return out
 
@@ -462,6 +483,8 @@ def main():
         b = result["batch"]
         log(f"batch: mixed {b['mixed']['value']} / fp64 {b['fp64']['value']} orientations/s "
             f"(x{b['mixed']['speedup_vs_sequential']} / x{b['fp64']['speedup_vs_sequential']} vs one at a time)")
+        log(f"batch, autotuned tiles: mixed {b['mixed']['autotuned']['value']} / fp64 "
+            f"{b['fp64']['autotuned']['value']} orientations/s (tiles {b['fp64']['autotuned']['tiles']})")
 
     if not a.no_cpu_baseline:
         log("cpu baseline")

@@ -19,6 +19,23 @@ from . import _lib
 from ._lib import check
 
 
+def autotune(dds):
+    """Fix each instance's column tile to the fastest for its own lattice,
+    L and precision (``bdda_plan_autotune``: the operator timed at each
+    candidate tile); done once per plan, L and precision.  Returns the tiles."""
+    tiles = []
+    for dd in dds:
+        key = (dd._B_dev.shape[1], dd.precision)
+        done = getattr(dd._plan, "_tuned", {})
+        if key not in done:
+            done[key] = dd._plan.autotune(*key)
+            dd._plan._tuned = done
+        else:
+            dd._plan.set_column_tile(done[key])
+        tiles.append(done[key])
+    return tiles
+
+
 def solve_batch(dds, nstreams=8):
     """Solve every set-up DiscreteDipoles in ``dds``; they must share solver,
     precision, tol, itermax and refine_every.  Per-instance results land on

@@ -81,6 +81,18 @@ class Plan:
             raise ValueError("column_tile: bad L")
         return lb
 
+    def set_column_tile(self, lb):
+        """Fix the column tile (0: automatic)."""
+        check(self.lib.bdda_plan_set_column_tile(self.h, int(lb)), "set_column_tile")
+
+    def autotune(self, L, precision="fp64"):
+        """Time the operator at the candidate column tiles for L columns and
+        keep the fastest; returns the tile."""
+        lb = ctypes.c_int(0)
+        check(self.lib.bdda_plan_autotune(self.h, int(L), _lib.PRECISION[precision], ctypes.byref(lb), _stream()),
+              "autotune")
+        return lb.value
+
     def green_tensor(self, lf, k):
         out = np.empty(self.m + (3, 3), dtype=np.complex128)
         check(self.lib.bdda_green_tensor(self.h, float(lf), float(k), ptr(out)), "green_tensor")

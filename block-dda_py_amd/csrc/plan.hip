@@ -131,6 +131,9 @@ struct bdda_plan {
   // iteration
   bdda::DevBuf dla, dla_misc;
   bdda::HostBuf h_sync;
+  // column tile set by bdda_plan_set_column_tile / bdda_plan_autotune (0:
+  // the rule of choose_lb)
+  int lb_force = 0;
   // optional per-pass kernel timing (HIP events recorded around each pass)
   bool profiling = false;
   std::vector<hipEvent_t> ev_pool;
@@ -171,10 +174,15 @@ template <> struct PrecTraits<float2> {
 // values; among the admissible tiles pick the one that pads L least, then
 // the 8-column tile (the register kernels have a compile-time version of it),
 // then the widest (longer contiguous runs per grid point).
-static int choose_lb(const bdda_plan* p, int L) {
+// Widest column tile every line-FFT kernel takes on this plan's lengths.
+static int lb_cap(const bdda_plan* p) {
   const int mmax = std::max(p->ax[0].m, std::max(p->ax[1].m, p->ax[2].m));
-  int cap = 4096 / (3 * mmax);
-  cap = std::max(1, std::min(cap, 16));
+  return std::max(1, std::min(4096 / (3 * mmax), 16));
+}
+
+static int choose_lb(const bdda_plan* p, int L) {
+  if (p->lb_force > 0) return p->lb_force;
+  const int cap = lb_cap(p);
   int best = 1;
   long long best_pad = (long long)L;
   for (int lb = cap; lb >= 1; --lb) {
@@ -1615,6 +1623,81 @@ int bdda_set_target(bdda_plan* p, const int64_t* address, int64_t N, const doubl
   p->N = N;
   p->target_ready = true;
   return 0;
+}
+
+int bdda_plan_set_column_tile(bdda_plan* p, int lb) {
+  if (int rc = check_plan(p)) return rc;
+  g_err.clear();
+  if (lb < 0 || lb > lb_cap(p)) {
+    set_error("bdda_plan_set_column_tile: tile must be 0 (automatic) or 1.." + std::to_string(lb_cap(p)));
+    return BDDA_E_ARG;
+  }
+  p->lb_force = lb;
+  return 0;
+}
+
+// Times the occupied operator (one warm-up, then three applications between
+// HIP events on the caller's stream) for the column tiles 4, 8 and the
+// automatic choice, within the plan's cap, and keeps the fastest.
+int bdda_plan_autotune(bdda_plan* p, int L, int precision, int* lb_out, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  g_err.clear();
+  if (!p->green_ready || !p->target_ready) {
+    set_error("bdda_plan_autotune: call bdda_green_init and bdda_set_target first");
+    return BDDA_E_STATE;
+  }
+  if (L < 1 || (precision != BDDA_PREC_FP64 && precision != BDDA_PREC_MIXED)) {
+    set_error("bdda_plan_autotune: bad arguments");
+    return BDDA_E_ARG;
+  }
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const int cap = lb_cap(p);
+  p->lb_force = 0;
+  std::vector<int> cand = {choose_lb(p, L)};
+  for (int lb : {8, 4})
+    if (lb <= cap && std::find(cand.begin(), cand.end(), lb) == cand.end()) cand.push_back(lb);
+  DevBuf S, AS;
+  const size_t bytes = (size_t)p->N * 3 * L * sizeof(double2);
+  if (int rc = S.ensure(bytes)) return rc;
+  if (int rc = AS.ensure(bytes)) return rc;
+  BDDA_HIP(hipMemsetAsync(S.p, 0, bytes, s));
+  hipEvent_t e0, e1;
+  BDDA_HIP(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    set_error("bdda_plan_autotune: hipEventCreate failed");
+    return BDDA_E_HIP;
+  }
+  int best = cand[0], rc = 0;
+  float best_ms = INFINITY;
+  for (int lb : cand) {
+    p->lb_force = lb;
+    auto apply = [&]() {
+      return precision == BDDA_PREC_MIXED
+                 ? apply_operator<float2, double2>(p, S.as<double2>(), AS.as<double2>(), L, true, s)
+                 : apply_operator<double2>(p, S.as<double2>(), AS.as<double2>(), L, true, s);
+    };
+    if ((rc = apply())) break;
+    if (hipEventRecord(e0, s) != hipSuccess) { rc = BDDA_E_HIP; break; }
+    for (int r = 0; r < 3 && !rc; ++r) rc = apply();
+    if (rc) break;
+    float ms = 0.0f;
+    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+      set_error("bdda_plan_autotune: event timing failed");
+      rc = BDDA_E_HIP;
+      break;
+    }
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = lb;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  p->lb_force = rc ? 0 : best;
+  if (!rc && lb_out) *lb_out = best;
+  return rc;
 }
 
 int bdda_block_mvp(bdda_plan* p, const void* S_dev, void* AS_dev, int L, void* stream) {

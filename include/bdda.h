@@ -75,6 +75,22 @@ int bdda_plan_fft_dims(const bdda_plan* plan, int* m_out);
  * -BDDA_E_ARG.  For measurement (byte counts); no reference counterpart. */
 int bdda_plan_column_tile(const bdda_plan* plan, int L);
 
+/* Fix the plan's column tile (1 .. the widest tile its FFT lengths allow,
+ * at most 16), or 0 to go back to the automatic choice above.  Returns
+ * BDDA_E_ARG for a tile out of range.  No reference counterpart (the
+ * reference's numpy FFTs have no tiles). */
+int bdda_plan_set_column_tile(bdda_plan* plan, int lb);
+
+/* Autotune the column tile for L columns in `precision`: times the occupied
+ * operator (one warm-up and three applications per candidate, on `stream`)
+ * at the tiles 4, 8 and the automatic choice, within the plan's limit, and
+ * fixes the fastest (*lb_out, nullable).  Needs bdda_green_init and
+ * bdda_set_target; allocates and frees one pair of (3N, L) blocks.  The
+ * result is the same operator to rounding at any tile.  For batched
+ * instances, whose lattices give every instance its own FFT lengths; no
+ * reference counterpart. */
+int bdda_plan_autotune(bdda_plan* plan, int L, int precision, int* lb_out, void* stream);
+
 /* Green tensor A'(r) on the doubled grid and its 3-D FFT, kept on device in
  * complex128 and complex64 (6 unique components, half spectrum).
  * Replaces build_interaction_tensor + fft_init (mvp_fft/mvp_fft.py:13-119). */

@@ -67,3 +67,40 @@ def test_batch_rejects_shared_plans(small_batch):
     dd = workloads.make_dipoles(w, geometry=g, precision="fp64")
     with pytest.raises(ValueError, match="own"):
         solve_batch([dd, dd])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_autotuned_tiles_keep_the_operator(small_batch, precision):
+    """bdda_plan_autotune fixes one of the candidate column tiles; the
+    operator at that tile equals the automatic tile's to rounding, and the
+    batch still converges to the tolerance with the tuned plans."""
+    from bdda_amd import workloads
+    from bdda_amd.batch import autotune, solve_batch
+    say(f"autotuned column tiles, {precision}")
+    dds = [workloads.make_dipoles(w, geometry=g, precision=precision) for w, g in small_batch]
+    g = torch.Generator(device="cuda").manual_seed(3)
+    for dd in dds:
+        plan, L = dd._plan, dd._B_dev.shape[1]
+        auto = plan.column_tile(L)
+        S = torch.randn(dd._B_dev.shape, dtype=torch.complex128, device="cuda", generator=g)
+        ref = plan.block_mvp(S)
+        lb = plan.autotune(L, precision)
+        assert lb in {auto, 4, 8} and plan.column_tile(L) == lb
+        got = plan.block_mvp(S)
+        assert float((got - ref).abs().max() / ref.abs().max()) < 1e-13
+        plan.set_column_tile(0)
+        assert plan.column_tile(L) == auto
+    tiles = autotune(dds)
+    assert all(dd._plan.column_tile(dd._B_dev.shape[1]) == t for dd, t in zip(dds, tiles))
+    solve_batch(dds, nstreams=3)
+    assert all(dd.converge and dd.true_residual < 1e-5 for dd in dds)
+
+
+def test_column_tile_out_of_range_is_refused(small_batch):
+    from bdda_amd import workloads
+    w, g = small_batch[0]
+    dd = workloads.make_dipoles(w, geometry=g, precision="fp64")
+    with pytest.raises(ValueError, match="tile"):
+        dd._plan.set_column_tile(17)
+    with pytest.raises(ValueError, match="tile"):
+        dd._plan.set_column_tile(-1)
