@@ -149,7 +149,7 @@ __global__ void __launch_bounds__(256) k_gram(const V* __restrict__ A, const V* 
 // loaded straight from the row-major blocks (a strip of one row is 256
 // contiguous bytes); result entry r of lane l is row l / 16 + 4 r, column
 // l % 16 of the 16 x 16 product (checked on the device against a host
-// product).  The next four rows are loaded while the current ones multiply.
+// product).  Sixteen rows are in flight while the previous sixteen multiply.
 // Partial layout as k_gram with one group.
 typedef double gram_d4 __attribute__((ext_vector_type(4)));
 // Row `row` of this lane's NT strips of A and B (zero beyond r1 or L).
@@ -164,7 +164,7 @@ __device__ __forceinline__ void gram_fetch(const double2* __restrict__ A, const 
   }
 }
 template <bool CONJ, int TL>
-__global__ void __launch_bounds__(256) k_gram_mfma(const double2* __restrict__ A, const double2* __restrict__ B,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_gram_mfma(const double2* __restrict__ A, const double2* __restrict__ B,
                                                    long long rows, int L, double2* __restrict__ partial) {
   constexpr int WS = TL / 2, NT = WS / 16;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -183,32 +183,46 @@ __global__ void __launch_bounds__(256) k_gram_mfma(const double2* __restrict__ A
       cr[i][j] = gram_d4{0.0, 0.0, 0.0, 0.0};
       ci[i][j] = gram_d4{0.0, 0.0, 0.0, 0.0};
     }
-  double2 a[NT], b[NT];
-  if (r0 < r1) gram_fetch<NT>(A, B, r0 + kr, r1, L, ja, lb, a, b);
-  for (long long k0 = r0; k0 < r1; k0 += 4) {
-    double2 ca[NT], cb[NT];
+  // PF groups of four rows in flight: the next 4 PF rows are loaded while
+  // the current ones multiply
+  constexpr int PF = 4;
+  double2 a[PF][NT], b[PF][NT];
 #pragma unroll
-    for (int i = 0; i < NT; ++i) {
-      ca[i] = a[i];
-      cb[i] = b[i];
+  for (int f = 0; f < PF; ++f) gram_fetch<NT>(A, B, r0 + 4 * f + kr, r1, L, ja, lb, a[f], b[f]);
+  for (long long k0 = r0; k0 < r1; k0 += 4 * PF) {
+    double2 ca[PF][NT], cb[PF][NT];
+#pragma unroll
+    for (int f = 0; f < PF; ++f)
+#pragma unroll
+      for (int i = 0; i < NT; ++i) {
+        ca[f][i] = a[f][i];
+        cb[f][i] = b[f][i];
+      }
+    if (k0 + 4 * PF < r1) {
+#pragma unroll
+      for (int f = 0; f < PF; ++f) gram_fetch<NT>(A, B, k0 + 4 * (PF + f) + kr, r1, L, ja, lb, a[f], b[f]);
     }
-    if (k0 + 4 < r1) gram_fetch<NT>(A, B, k0 + 4 + kr, r1, L, ja, lb, a, b);
     // conj:  Re += ar br + ai bi, Im += ar bi - ai br
     // plain: Re += ar br - ai bi, Im += ar bi + ai br
 #pragma unroll
-    for (int i = 0; i < NT; ++i)
+    for (int f = 0; f < PF; ++f) {
 #pragma unroll
-      for (int j = 0; j < NT; ++j) {
-        cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[i].x, cb[j].x, cr[i][j], 0, 0, 0);
-        ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[i].x, cb[j].y, ci[i][j], 0, 0, 0);
-      }
+      for (int i = 0; i < NT; ++i)
 #pragma unroll
-    for (int i = 0; i < NT; ++i)
+        for (int j = 0; j < NT; ++j) {
+          cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[f][i].x, cb[f][j].x, cr[i][j], 0, 0, 0);
+          ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[f][i].x, cb[f][j].y, ci[i][j], 0, 0, 0);
+        }
 #pragma unroll
-      for (int j = 0; j < NT; ++j) {
-        cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(CONJ ? ca[i].y : -ca[i].y, cb[j].y, cr[i][j], 0, 0, 0);
-        ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(CONJ ? -ca[i].y : ca[i].y, cb[j].x, ci[i][j], 0, 0, 0);
-      }
+      for (int i = 0; i < NT; ++i)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+          cr[i][j] =
+              __builtin_amdgcn_mfma_f64_16x16x4f64(CONJ ? ca[f][i].y : -ca[f][i].y, cb[f][j].y, cr[i][j], 0, 0, 0);
+          ci[i][j] =
+              __builtin_amdgcn_mfma_f64_16x16x4f64(CONJ ? -ca[f][i].y : ca[f][i].y, cb[f][j].x, ci[i][j], 0, 0, 0);
+        }
+    }
   }
   double2* out = partial + ((long long)blockIdx.x * gridDim.y + blockIdx.y) * (TL * TL);
 #pragma unroll
@@ -408,94 +422,17 @@ __global__ void __launch_bounds__(256) k_update(V* Y, const V* X, double2 a, con
   }
 }
 
-// The same update on the float64 matrix cores, for complex128 blocks with
-// 32 < L <= 64 (one 64-column tile, K = L <= 64).  C (L x 64) is staged in
-// LDS once per block; the block then walks slabs of 64 rows, wave w taking
-// rows 16 w .. +16 against all 64 columns.  Step s takes coefficients
-// j = 4 s .. 4 s + 3: lane l loads row l % 16, coefficient 4 s + l / 16 of
-// W1 + w2c W2 and runs four 16 x 16 column tiles through
-// v_mfma_f64_16x16x4f64, real and imaginary parts in separate accumulators.  Entry r of lane l of a tile is row l / 16 + 4 r,
-// column l % 16; the epilogue adds a X and e E there.  Sums over j run in a
-// fixed order (the hardware's within each step of four), so repeated
-// updates agree bit for bit.
-template <int KS>
-__global__ void __launch_bounds__(256) k_update_mfma(double2* Y, const double2* X, double2 a, const double2* W1,
-                                                     const double2* W2, double2 w2c,
-                                                     const double2* __restrict__ C, const double2* E, double2 e,
-                                                     long long rows, int L, const double2* sc) {
-  __shared__ double2 Cs[4 * KS][64];
-  if (sc) {
-    w2c = sc[0];
-    e = sc[1];
-  }
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int kr = lane >> 4, cc = lane & 15;
-  for (int idx = tid; idx < 4 * KS * 64; idx += 256) {
-    const int j = idx / 64, l = idx % 64;
-    Cs[j][l] = (j < L && l < L) ? C[(long long)j * L + l] : make_double2(0.0, 0.0);
-  }
-  __syncthreads();
-  for (long long r0 = blockIdx.x * 64LL; r0 < rows; r0 += 64LL * gridDim.x) {
-    const long long row = r0 + 16 * w + cc;  // this lane's W row
-    gram_d4 yr[4], yi[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      yr[t] = gram_d4{0.0, 0.0, 0.0, 0.0};
-      yi[t] = gram_d4{0.0, 0.0, 0.0, 0.0};
-    }
-#pragma unroll 4
-    for (int st = 0; st < KS; ++st) {
-      const int j = 4 * st + kr;
-      double2 v = make_double2(0.0, 0.0);
-      if (row < rows && j < L) {
-        v = W1[row * L + j];
-        if (W2) {
-          const double2 u = W2[row * L + j];
-          v.x += w2c.x * u.x - w2c.y * u.y;
-          v.y += w2c.x * u.y + w2c.y * u.x;
-        }
-      }
-      const double2 ct[4] = {Cs[j][cc], Cs[j][16 + cc], Cs[j][32 + cc], Cs[j][48 + cc]};
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        yr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(v.x, ct[t].x, yr[t], 0, 0, 0);
-        yi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(v.x, ct[t].y, yi[t], 0, 0, 0);
-      }
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        yr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-v.y, ct[t].y, yr[t], 0, 0, 0);
-        yi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(v.y, ct[t].x, yi[t], 0, 0, 0);
-      }
-    }
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const long long orow = r0 + 16 * w + kr + 4 * r;
-        const int l = 16 * t + cc;
-        if (orow >= rows || l >= L) continue;
-        const long long o = orow * L + l;
-        double2 y = make_double2(yr[t][r], yi[t][r]);
-        if (X) {
-          const double2 x = X[o];
-          y.x += a.x * x.x - a.y * x.y;
-          y.y += a.x * x.y + a.y * x.x;
-        }
-        if (E) {
-          const double2 ev = E[o];
-          y.x += e.x * ev.x - e.y * ev.y;
-          y.y += e.x * ev.y + e.y * ev.x;
-        }
-        Y[o] = y;
-      }
-  }
-}
-
-// The update for complex128 blocks with L <= 16 NT (NT = 1 or 2) on the
-// float64 matrix cores, optionally with the Gram matrix op(Y)^T Y of the
-// block it writes (GRAM 1: Y^H Y, 2: Y^T Y), so that the Gram needs no second
-// pass over Y.  Wave w of a block takes rows 16 w .. +16 of each 64-row slab
-// against all 16 NT columns, as in k_update_mfma.  Entry r of tile t of lane
+// The update for complex128 blocks with L <= 16 NT (NT = 1, 2 or 4) on the
+// float64 matrix cores, optionally (NT <= 2) with the Gram matrix op(Y)^T Y
+// of the block it writes (GRAM 1: Y^H Y, 2: Y^T Y), so that the Gram needs
+// no second pass over Y.  C (L x L) is staged in LDS once per block; the
+// block walks 64-row slabs, wave w taking rows 16 w .. +16 against all 16 NT
+// columns.  Step st takes coefficients j = 4 st .. 4 st + 3: lane l holds
+// row l % 16, coefficient 4 st + l / 16 of W1 + w2c W2 (the A operand) and
+// C[4 st + l / 16][16 t + l % 16] (the B operand) of v_mfma_f64_16x16x4f64,
+// real and imaginary parts in separate accumulators; sums over j run in a
+// fixed order.  Every load of a slab is issued before the first is used.
+// Entry r of tile t of lane
 // l is Y[row kr + 4 r][column 16 t + cc] (kr = l / 16, cc = l % 16), which is
 // exactly the operand layout of v_mfma_f64_16x16x4f64 over the four rows
 // kr + 4 r (A[i][k] = op(Y[k][16 J + i]) from lane i + 16 k, B[k][j] =
@@ -511,8 +448,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT == 
                                                      const double2* __restrict__ C, const double2* E, double2 e,
                                                      long long rows, int L, const double2* sc,
                                                      double2* __restrict__ partial) {
-  constexpr int TL = 16 * NT, KS = TL / 4, NP = NT * (NT + 1) / 2;
+  constexpr int TL = 16 * NT, KS = TL / 4, NP = NT * (NT + 1) / 2, CH = NT == 4 ? 2 : 4;
   constexpr bool CONJ = GRAM == 1;
+  static_assert(GRAM == 0 || NT <= 2, "the Gram accumulators of wider tiles do not fit the registers");
   __shared__ double2 Cs[TL][TL];
   if (sc) {
     w2c = sc[0];
@@ -537,49 +475,77 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT == 
     // (always valid) addresses; out-of-range entries are zeroed afterwards.
     const long long row = r0 + 16 * w + cc;  // this lane's W row
     const long long rowc = row < rows ? row : rows - 1;
-    double2 v[KS];
-#pragma unroll
-    for (int st = 0; st < KS; ++st) {
-      const int j = 4 * st + kr;
-      v[st] = W1[rowc * L + (j < L ? j : L - 1)];
-    }
-    if (has_w2) {
-      double2 u[KS];
-#pragma unroll
-      for (int st = 0; st < KS; ++st) {
-        const int j = 4 * st + kr;
-        u[st] = W2[rowc * L + (j < L ? j : L - 1)];
-      }
-#pragma unroll
-      for (int st = 0; st < KS; ++st) {
-        v[st].x += w2c.x * u[st].x - w2c.y * u[st].y;
-        v[st].y += w2c.x * u[st].y + w2c.y * u[st].x;
-      }
-    }
-#pragma unroll
-    for (int st = 0; st < KS; ++st)
-      if (!(row < rows && 4 * st + kr < L)) v[st] = make_double2(0.0, 0.0);
-    // the epilogue's a X + e E at entries (row kr + 4 r, column 16 t + cc)
+    // the epilogue's a X + e E at entries (row kr + 4 r, column 16 t + cc),
+    // loaded after the first chunk of W (in flight during its products)
     double2 xv[NT][4];
-#pragma unroll
-    for (int t = 0; t < NT; ++t)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) xv[t][r] = make_double2(0.0, 0.0);
     auto entry = [&](int t, int r) -> long long {
       long long orow = r0 + 16 * w + kr + 4 * r;
       orow = orow < rows ? orow : rows - 1;
       const int l = 16 * t + cc;
       return orow * L + (l < L ? l : L - 1);
     };
-    if (has_x) {
+    gram_d4 yr[NT], yi[NT];
 #pragma unroll
-      for (int t = 0; t < NT; ++t)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const double2 x = X[entry(t, r)];
-          xv[t][r] = make_double2(a.x * x.x - a.y * x.y, a.x * x.y + a.y * x.x);
-        }
+    for (int t = 0; t < NT; ++t) {
+      yr[t] = gram_d4{0.0, 0.0, 0.0, 0.0};
+      yi[t] = gram_d4{0.0, 0.0, 0.0, 0.0};
     }
+    // W in chunks of CH coefficient steps: a chunk's loads are issued
+    // together, one chunk ahead of its products
+    auto load_chunk = [&](int c0, double2 (&v)[CH]) {
+#pragma unroll
+      for (int q = 0; q < CH; ++q) {
+        const int j = 4 * (c0 + q) + kr;
+        v[q] = W1[rowc * L + (j < L ? j : L - 1)];
+      }
+      if (has_w2) {
+        double2 u[CH];
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+          const int j = 4 * (c0 + q) + kr;
+          u[q] = W2[rowc * L + (j < L ? j : L - 1)];
+        }
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+          v[q].x += w2c.x * u[q].x - w2c.y * u[q].y;
+          v[q].y += w2c.x * u[q].y + w2c.y * u[q].x;
+        }
+      }
+    };
+    double2 va[CH], vb[CH];
+    load_chunk(0, va);
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) xv[t][r] = has_x ? X[entry(t, r)] : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int c0 = 0; c0 < KS; c0 += CH) {
+      if (c0 + CH < KS) load_chunk(c0 + CH, vb);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int q = 0; q < CH; ++q) {
+        const int j = 4 * (c0 + q) + kr;
+        const double2 v = (row < rows && j < L) ? va[q] : make_double2(0.0, 0.0);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          const double2 ct = Cs[j][16 * t + cc];
+          yr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(v.x, ct.x, yr[t], 0, 0, 0);
+          yi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(v.x, ct.y, yi[t], 0, 0, 0);
+          yr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-v.y, ct.y, yr[t], 0, 0, 0);
+          yi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(v.y, ct.x, yi[t], 0, 0, 0);
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int q = 0; q < CH; ++q) va[q] = vb[q];
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double2 x = xv[t][r];
+        xv[t][r] = make_double2(a.x * x.x - a.y * x.y, a.x * x.y + a.y * x.x);
+      }
     if (has_e) {
 #pragma unroll
       for (int t = 0; t < NT; ++t)
@@ -589,24 +555,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT == 
           xv[t][r].x += e.x * ee.x - e.y * ee.y;
           xv[t][r].y += e.x * ee.y + e.y * ee.x;
         }
-    }
-    gram_d4 yr[NT], yi[NT];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      yr[t] = gram_d4{0.0, 0.0, 0.0, 0.0};
-      yi[t] = gram_d4{0.0, 0.0, 0.0, 0.0};
-    }
-#pragma unroll
-    for (int st = 0; st < KS; ++st) {
-      const int j = 4 * st + kr;
-#pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const double2 ct = Cs[j][16 * t + cc];
-        yr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[st].x, ct.x, yr[t], 0, 0, 0);
-        yi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[st].x, ct.y, yi[t], 0, 0, 0);
-        yr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-v[st].y, ct.y, yr[t], 0, 0, 0);
-        yi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[st].y, ct.x, yi[t], 0, 0, 0);
-      }
     }
     double2 yv[NT][4];
 #pragma unroll
@@ -702,10 +650,15 @@ static void launch_update_gram_nt(double2* Y, const double2* X, double2 a, const
 void launch_update_gram(double2* Y, const double2* X, double2 a, const double2* W1, const double2* W2, double2 w2c,
                         const double2* C, const double2* E, double2 e, long long rows, int L, int gram,
                         double2* partial, double2* out, hipStream_t s, const double2* sc) {
-  if (L <= 16)
+  if (L <= 16) {
     launch_update_gram_nt<1>(Y, X, a, W1, W2, w2c, C, E, e, rows, L, gram, partial, out, s, sc);
-  else
+  } else if (L <= 32) {
     launch_update_gram_nt<2>(Y, X, a, W1, W2, w2c, C, E, e, rows, L, gram, partial, out, s, sc);
+  } else {  // L <= 64, the update alone
+    const int g = update_gram_parts(rows);
+    hipLaunchKernelGGL((k_update_gram<4, 0>), dim3(g), dim3(256), 0, s, Y, X, a, W1, W2, w2c, C, E, e, rows, L, sc,
+                       partial);
+  }
 }
 
 template <typename V, int TL>
@@ -713,15 +666,6 @@ static void launch_update_tile(V* Y, const V* X, double2 a, const V* W1, const V
                                const double2* C, const V* E, double2 e, long long rows, int L, hipStream_t s,
                                const double2* sc) {
   constexpr int RB = 4096 / TL;
-  if constexpr (std::is_same<V, double2>::value && TL == 64) {
-    if (L <= 64) {  // one column tile: the matrix-core kernel
-      const long long slabs = (rows + 63) / 64;
-      const int g = (int)min(slabs, 1024LL);
-      hipLaunchKernelGGL((k_update_mfma<16>), dim3(g), dim3(256), 0, s, Y, X, a, W1, W2, w2c, C, E, e, rows, L,
-                         sc);
-      return;
-    }
-  }
   const long long slabs = (rows + RB - 1) / RB;
   const int g = (int)min(slabs, 4096LL);
   dim3 grid(g, (L + TL - 1) / TL);
@@ -732,7 +676,7 @@ template <typename V>
 void launch_update(V* Y, const V* X, double2 a, const V* W1, const V* W2, double2 w2c, const double2* C,
                    const V* E, double2 e, long long rows, int L, hipStream_t s, const double2* sc) {
   if constexpr (std::is_same<V, double2>::value) {
-    if (L <= kUpdateGramMaxL) {  // the matrix-core kernel without its Gram
+    if (L <= 64) {  // the matrix-core kernel without its Gram
       launch_update_gram(Y, X, a, W1, W2, w2c, C, E, e, rows, L, 0, nullptr, nullptr, s, sc);
       return;
     }
