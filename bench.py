@@ -223,6 +223,28 @@ def warm_leg(dd, steps, torch, t_fp64, S_fw64, S_bk64):
     return out
 
 
+def groups_leg(dd, steps, torch, t_fp64, t_mixed, groups=4):
+    """The same float64 and mixed solves with the operator's column tiles in
+    `groups` groups on staggered streams (bdda_set_operator_groups): one
+    group's x pass overlaps the next group's z and y passes.  The results
+    are bit-identical to the plain operator's."""
+    from bdda_amd import _lib
+    lib = _lib.load()
+    X_ref = dd.X_device.clone()
+    out = {"groups": groups, "note": "opt-in (bdda_set_operator_groups); the headline value and the roofline "
+                                     "use the plain operator"}
+    _lib.check(lib.bdda_set_operator_groups(groups))
+    try:
+        for prec, t_ref in (("fp64", t_fp64), ("mixed", t_mixed)):
+            dtg, _ = timed_solves(dd, prec, steps, 1, torch, profile_steps=0)
+            out[prec] = {"ms_per_step": round(dtg / steps * 1e3, 2), "speedup_vs_plain": round(t_ref / (dtg / steps), 3)}
+            if prec == "mixed":
+                out[prec]["identical_to_plain"] = bool(torch.equal(dd.X_device, X_ref))
+    finally:
+        lib.bdda_set_operator_groups(1)
+    return out
+
+
 def trimmed_leg(wl, geometry, steps, torch, t_fp64, S_fw64, S_bk64):
     """The benchmark particle without the 64^3 padding: the solver's cuboid
     is then the smallest box of lattice cells holding the occupied sites
@@ -457,6 +479,12 @@ def main():
             "passes_c64": mp,
         }
         log(f"mixed: {dtm / a.steps * 1e3:.1f} ms/solve")
+
+    if not a.no_mixed:
+        result["operator_groups"] = groups_leg(dd, a.steps, torch, dt / a.steps, dtm / a.steps)
+        og = result["operator_groups"]
+        log(f"operator in {og['groups']} column groups: fp64 {og['fp64']['ms_per_step']} ms, "
+            f"mixed {og['mixed']['ms_per_step']} ms")
 
     if not a.no_mixed and not a.no_trim:
         log("same particle on its occupied extent")

@@ -28,6 +28,7 @@ _SIGNATURES = {
     "bdda_last_error": ([], ctypes.c_char_p),
     "bdda_version": ([], _i),
     "bdda_set_fft_path": ([_i], _i),
+    "bdda_set_operator_groups": ([_i], _i),
     "bdda_set_la_path": ([_i], _i),
     "bdda_plan_column_tile": ([_vp, _i], _i),
     "bdda_plan_set_column_tile": ([_vp, _i], _i),

@@ -464,7 +464,7 @@ __global__ void __launch_bounds__(512) k_line_pass(LinePass lp) {
   // issued before any LDS store
   V v[VPT];
   if (is_scatter<MODE>() || MODE == PASS_CUBOID_Z) {
-    const int c = q / lp.Lb, l = lo * lp.Lb + q % lp.Lb;
+    const int c = q / lp.Lb, l = lp.col0 + lo * lp.Lb + q % lp.Lb;
     long long mrow[VPT];
 #pragma unroll
     for (int i = 0; i < VPT; ++i) {
@@ -513,7 +513,7 @@ __global__ void __launch_bounds__(512) k_line_pass(LinePass lp) {
     using T = typename RealOf<V>::T;
     using TB = typename RealOf<BV>::T;
     const T sc = (T)lp.scale;
-    const int c = q / lp.Lb, l = lo * lp.Lb + q % lp.Lb;
+    const int c = q / lp.Lb, l = lp.col0 + lo * lp.Lb + q % lp.Lb;
     if (!active || l >= lp.L) return;
     V* __restrict__ out = reinterpret_cast<V*>(lp.AS);
     long long row[VPT];
@@ -693,7 +693,7 @@ __attribute__((amdgpu_waves_per_eu(reg2_waves_per_simd<V, R2, MODE>()))) k_reg2(
     V a[RMAX];
     const V* __restrict__ src = reinterpret_cast<const V*>(lp.src) + (long long)lo * lp.lo_src;
     if (is_scatter<MODE>() || MODE == PASS_CUBOID_Z) {
-      const int c = q / Lb, l = lo * Lb + q % Lb;
+      const int c = q / Lb, l = lp.col0 + lo * Lb + q % Lb;
       long long mrow[R1];
 #pragma unroll
       for (int j1 = 0; j1 < R1; ++j1) {
@@ -800,7 +800,7 @@ __attribute__((amdgpu_waves_per_eu(reg2_waves_per_simd<V, R2, MODE>()))) k_reg2(
         using T = typename RealOf<V>::T;
         using TB = typename RealOf<BV>::T;
         const T sc = (T)lp.scale;
-        const int c = q / Lb, l = lo * Lb + q % Lb;
+        const int c = q / Lb, l = lp.col0 + lo * Lb + q % Lb;
         if (l < lp.L) {
           V* __restrict__ out = reinterpret_cast<V*>(lp.AS);
           // rows 3m + c of the output cells (m from the prefetched index)

@@ -83,6 +83,7 @@ struct LinePass {
   const unsigned char* lmask;
   int nyz_y, nz;        // cuboid Ny, Nz (z-line points are (ix, iy))
   int L, Lb;            // total columns, column tile
+  int col0;             // first column of this launch's tiles (scatter / gather blocks)
   int nlo;              // number of column tiles (grid.y)
   const void* S;        // occupied block (3N, L) (scatter source / diag term)
   void* AS;             // occupied output (3N, L)

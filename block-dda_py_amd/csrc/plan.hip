@@ -139,8 +139,19 @@ struct bdda_plan {
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   std::vector<int> ev_pass;  // pass id of each recorded pair (precision * 5 + pass)
+  // overlapped operator: column groups 1.. run on their own streams, each
+  // started once the previous group's y forward pass is done
+  static constexpr int kMaxGroups = 4;
+  hipStream_t grp_stream[kMaxGroups] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t grp_after_y[kMaxGroups] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t grp_done[kMaxGroups] = {nullptr, nullptr, nullptr, nullptr};
   ~bdda_plan() {
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+    for (int g = 0; g < kMaxGroups; ++g) {
+      if (grp_stream[g]) (void)hipStreamDestroy(grp_stream[g]);
+      if (grp_after_y[g]) (void)hipEventDestroy(grp_after_y[g]);
+      if (grp_done[g]) (void)hipEventDestroy(grp_done[g]);
+    }
   }
 };
 
@@ -174,6 +185,11 @@ template <> struct PrecTraits<float2> {
 // values; among the admissible tiles pick the one that pads L least, then
 // the 8-column tile (the register kernels have a compile-time version of it),
 // then the widest (longer contiguous runs per grid point).
+// Column groups of the occupied operator (bdda_set_operator_groups): 1 runs
+// the five passes once over every column tile; G > 1 splits the tiles into
+// G groups on staggered streams.
+std::atomic<int> g_op_groups{1};
+
 // Widest column tile every line-FFT kernel takes on this plan's lengths.
 static int lb_cap(const bdda_plan* p) {
   const int mmax = std::max(p->ax[0].m, std::max(p->ax[1].m, p->ax[2].m));
@@ -310,6 +326,7 @@ static int apply_operator(bdda_plan* p, const BV* S, BV* AS, int L, bool occupie
     for (int i = 0; i < kMaxRadix; ++i) lp.rad[i] = p->ax[a].rad[i];
     lp.tw = PrecTraits<V>::tw(p, a);
   };
+  LinePass lps[5];
   // P1: z forward with scatter (points (ix, iy))
   {
     LinePass lp = base;
@@ -329,8 +346,7 @@ static int apply_operator(bdda_plan* p, const BV* S, BV* AS, int L, bool occupie
     lp.lo_dst = t1_lo;
     lp.dst = T1.p;
     lp.order_pe = 0;
-    launch_timed<V>(p, lp, 0, s);
-    BDDA_CHECK_LAUNCH();
+    lps[0] = lp;
   }
   // P2: y forward (points (ix, kz)), T1 -> T2
   {
@@ -352,8 +368,7 @@ static int apply_operator(bdda_plan* p, const BV* S, BV* AS, int L, bool occupie
     lp.src = T1.p;
     lp.dst = p->T2.p;
     lp.order_pe = 1;
-    launch_timed<V>(p, lp, 1, s);
-    BDDA_CHECK_LAUNCH();
+    lps[1] = lp;
   }
   // P3: x forward * Green * x inverse (points (ky, kz)), in place on T2
   {
@@ -373,8 +388,7 @@ static int apply_operator(bdda_plan* p, const BV* S, BV* AS, int L, bool occupie
     lp.src = p->T2.p;
     lp.dst = p->T2.p;
     lp.order_pe = 1;
-    launch_timed<V>(p, lp, 2, s);
-    BDDA_CHECK_LAUNCH();
+    lps[2] = lp;
   }
   // P4: y inverse (points (ix, kz)), T2 -> T1
   {
@@ -398,8 +412,7 @@ static int apply_operator(bdda_plan* p, const BV* S, BV* AS, int L, bool occupie
     lp.src = p->T2.p;
     lp.dst = T1.p;
     lp.order_pe = 1;
-    launch_timed<V>(p, lp, 3, s);
-    BDDA_CHECK_LAUNCH();
+    lps[3] = lp;
   }
   // P5: z inverse with crop and gather/epilogue (points (ix, iy))
   {
@@ -422,9 +435,42 @@ static int apply_operator(bdda_plan* p, const BV* S, BV* AS, int L, bool occupie
     lp.src = T1.p;
     lp.scale = 1.0 / ((double)mx * my * mz);
     lp.order_pe = 0;
-    launch_timed<V>(p, lp, 4, s);
-    BDDA_CHECK_LAUNCH();
+    lps[4] = lp;
   }
+  // The passes of column tiles [t0, t1) on stream st (the T1 / T2 blocks of
+  // those tiles, and their columns of S and AS); after_y (nullable) is
+  // recorded once their y forward pass is queued.
+  auto run_tiles = [&](int t0, int t1, hipStream_t st, hipEvent_t after_y) -> int {
+    for (int k = 0; k < 5; ++k) {
+      LinePass lp = lps[k];
+      lp.nlo = t1 - t0;
+      lp.col0 = t0 * Lb;
+      if (lp.src) lp.src = reinterpret_cast<const V*>(lp.src) + (long long)t0 * lp.lo_src;
+      if (lp.dst) lp.dst = reinterpret_cast<V*>(lp.dst) + (long long)t0 * lp.lo_dst;
+      launch_timed<V>(p, lp, k, st);
+      BDDA_CHECK_LAUNCH();
+      if (k == 1 && after_y) BDDA_HIP(hipEventRecord(after_y, st));
+    }
+    return 0;
+  };
+  // Column groups: the x pass of one group (in-core bound) overlaps the
+  // z and y passes of the next (HBM bound).  Per-pass profiling runs the
+  // plain sequence.
+  const int G = occupied && !p->profiling ? std::min(std::min(g_op_groups.load(), nlo), bdda_plan::kMaxGroups) : 1;
+  if (G <= 1) return run_tiles(0, nlo, s, nullptr);
+  for (int g = 0; g < G; ++g) {
+    if (!p->grp_after_y[g]) BDDA_HIP(hipEventCreateWithFlags(&p->grp_after_y[g], hipEventDisableTiming));
+    if (!p->grp_done[g]) BDDA_HIP(hipEventCreateWithFlags(&p->grp_done[g], hipEventDisableTiming));
+    if (g > 0 && !p->grp_stream[g]) BDDA_HIP(hipStreamCreateWithFlags(&p->grp_stream[g], hipStreamNonBlocking));
+  }
+  for (int g = 0; g < G; ++g) {
+    const int t0 = (int)((long long)nlo * g / G), t1 = (int)((long long)nlo * (g + 1) / G);
+    hipStream_t st = g == 0 ? s : p->grp_stream[g];
+    if (g > 0) BDDA_HIP(hipStreamWaitEvent(st, p->grp_after_y[g - 1], 0));
+    if ((rc = run_tiles(t0, t1, st, p->grp_after_y[g]))) return rc;
+    if (g > 0) BDDA_HIP(hipEventRecord(p->grp_done[g], st));
+  }
+  for (int g = 1; g < G; ++g) BDDA_HIP(hipStreamWaitEvent(s, p->grp_done[g], 0));
   return 0;
 }
 
@@ -1318,6 +1364,15 @@ int bdda_set_la_path(int path) {
     return BDDA_E_ARG;
   }
   g_la_path = path;
+  return 0;
+}
+
+int bdda_set_operator_groups(int groups) {
+  if (groups < 1 || groups > bdda_plan::kMaxGroups) {
+    set_error("bdda_set_operator_groups: groups must be 1.." + std::to_string(bdda_plan::kMaxGroups));
+    return BDDA_E_ARG;
+  }
+  g_op_groups.store(groups);
   return 0;
 }
 

@@ -54,6 +54,15 @@ int bdda_version(void);
  * cross-check the kernels. */
 int bdda_set_fft_path(int path);
 
+/* Column groups of the occupied operator, for the whole process: 1 (default)
+ * runs each FFT pass once over all column tiles; G (2..4) splits the tiles
+ * into G groups, each on its own stream, started once the previous group's
+ * y forward pass is done, so that one group's x pass overlaps the next
+ * group's z and y passes.  The same arithmetic per column: results are
+ * bit-identical to G = 1.  BDDA_E_ARG outside 1..4.  No reference
+ * counterpart. */
+int bdda_set_operator_groups(int groups);
+
 /* Where the solvers' L x L recurrence algebra runs, for the whole process:
  * 0 (default) on the host after every Gram matrix (LAPACK-order LU with
  * partial pivoting, Cholesky, triangular inverse; 3-4 waits per iteration);
