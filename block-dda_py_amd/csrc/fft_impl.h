@@ -1156,11 +1156,179 @@ k_xconv(LinePass lp) {
   xconv_point<V, R1, R2, LB>(lp, buf, tws, a, pt, lo, tid);
 }
 
+// ---------------------------------------------------------------------------
+// x pass in two halves of the spectrum: one point (ky, kz) and a compile-time
+// tile of LB orientations per block (Q = 3 LB columns), m = R1 R2 with R1
+// even, HK = R1 / 2.  With x = x[R2 j1 + j2] (j1 < HK: every Goodman line is
+// half zero) and k = k1 + R1 k2, the first forward step Y[k1][j2] =
+// sum_j1 x W_R1^(j1 k1) splits by the parity of k1: Y[2k] = DFT_HK(x)[k],
+// Y[2k+1] = DFT_HK(x_j1 W_R1^j1)[k].  The last inverse step splits the same
+// way: for n1 < HK, x_out[R2 n1 + n2] = A[n1] + W_R1^(-n1) B[n1], with A and
+// B the inverse DFT_HKs over k of the rows V[2k][n2] and V[2k+1][n2].  So the
+// block runs the rest of the pipeline once per parity p (the HK R2 spectral
+// rows k1 = 2k + p) through an LDS tile of m/2 rows instead of m:
+//  step 1   thread (q, w = j2 < R2): both DFT_HKs of its HK inputs, twiddled
+//           by W_m^(k1 j2); the even half to LDS row k R2 + w, the odd half
+//           kept in registers for p = 1.
+//  step 2   thread (q, w = k < HK): DFT_R2 over j2 of rows w R2 + j2,
+//           written back to rows w R2 + k2 (spectral index k1 + R1 k2,
+//           k1 = 2 w + p).
+//  multiply thread (row r, group g): the row's six Green entries, y_c =
+//           sum_c' G_cc' x_c' for orientations g, g + G, ...
+//  inverse  thread (q, w = k < HK): inverse DFT_R2 over k2, twiddle
+//           W_m^(-n2 k1), rows w R2 + n2; then thread (q, w = n2 < R2):
+//           inverse DFT_HK over k of rows k R2 + w gives A (p = 0, kept) or
+//           B, and the outputs A + W_R1^(-n1) B, n1 < HK, are stored.
+// Half the LDS of k_xconv, so about twice the lines in flight per CU, and
+// the first and last steps keep every thread busy.
+template <int R1, int R2, int LB>
+constexpr int xsplit_threads() {
+  return 3 * LB * (R2 > R1 / 2 ? R2 : R1 / 2);
+}
+// Waves per SIMD the split x pass is compiled for: four (<= 128 VGPRs) keeps
+// the blocks that its LDS allows resident; the complex128 instances with a
+// second radix of 12 or more need more registers than that without spills.
+template <typename V, int R2>
+constexpr int xsplit_waves_per_simd() {
+  return (sizeof(V) == sizeof(double2) && R2 >= 12) ? 2 : (sizeof(V) == sizeof(double2) && R2 >= 10 ? 3 : 4);
+}
+template <typename V, int R1, int R2, int LB>
+__global__ void __launch_bounds__((xsplit_threads<R1, R2, LB>()))
+__attribute__((amdgpu_waves_per_eu(xsplit_waves_per_simd<V, R2>()))) k_xconv_split(LinePass lp) {
+  constexpr int HK = R1 / 2, M = R1 * R2, H = HK * R2, Q = 3 * LB, S = Q | 1;
+  constexpr int NTHR = xsplit_threads<R1, R2, LB>();
+  constexpr int G = NTHR / H;  // orientation groups of the multiply
+  static_assert(R1 % 2 == 0 && G >= 1, "even first radix; a thread per multiply row");
+  using T = typename RealOf<V>::T;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  V* buf = reinterpret_cast<V*>(smem_raw);
+  V* tws = buf + H * S;
+  const int tid = threadIdx.x, q = tid % Q, w = tid / Q;
+  const bool s1 = w < R2, s2 = w < HK;
+  int pt, lo;
+  xconv_item(lp, blockIdx.x, &pt, &lo);
+  if (pt >= lp.npts) return;
+  V e[HK];
+  xconv_load<V, R1, R2, LB>(lp, pt, lo, q, w, e);
+  for (int i = tid; i < M; i += NTHR) tws[i] = reinterpret_cast<const V*>(lp.tw)[i];
+  // step 1: odd half from x_j1 W_R1^j1 (compile-time twiddles), even half in place
+  constexpr TwiddleR<R1> wr{};
+  V o[HK];
+#pragma unroll
+  for (int j = 0; j < HK; ++j) {
+    const T c = (T)wr.c[j], sn = (T)wr.s[j];
+    o[j] = j == 0 ? e[j] : cx<V>(e[j].x * c + e[j].y * sn, e[j].y * c - e[j].x * sn);
+  }
+  Dft<HK, false, V>::run(e);
+  Dft<HK, false, V>::run(o);
+  __syncthreads();  // twiddles staged
+  if (s1) {
+#pragma unroll
+    for (int k = 0; k < HK; ++k) {
+      buf[(k * R2 + w) * S + q] = k == 0 ? e[0] : cmul(e[k], tws[2 * k * w]);
+      if (w != 0) o[k] = cmul(o[k], tws[(2 * k + 1) * w]);
+    }
+  }
+  const int r = tid % H, grp = tid / H;
+  V A[HK];
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    if (p == 1 && s1) {
+#pragma unroll
+      for (int k = 0; k < HK; ++k) buf[(k * R2 + w) * S + q] = o[k];
+    }
+    __syncthreads();
+    // step 2
+    V a[R2 > HK ? R2 : HK];
+    if (s2) {
+#pragma unroll
+      for (int j2 = 0; j2 < R2; ++j2) a[j2] = buf[(w * R2 + j2) * S + q];
+      Dft<R2, false, V>::run(a);
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) buf[(w * R2 + k2) * S + q] = a[k2];
+    }
+    // the Green row of this thread's multiply row, in flight over the barrier
+    V gr[6];
+    if (grp < G) green_row<V>(lp, pt, 2 * (r / R2) + p + R1 * (r % R2), gr);
+    __syncthreads();
+    // multiply
+    if (grp < G) {
+      V* row = buf + r * S;
+#pragma unroll
+      for (int li = grp; li < LB; li += G) {
+        const V x0 = row[li], x1 = row[LB + li], x2 = row[2 * LB + li];
+        row[li] = cadd(cadd(cmul(gr[0], x0), cmul(gr[1], x1)), cmul(gr[2], x2));
+        row[LB + li] = cadd(cadd(cmul(gr[1], x0), cmul(gr[3], x1)), cmul(gr[4], x2));
+        row[2 * LB + li] = cadd(cadd(cmul(gr[2], x0), cmul(gr[4], x1)), cmul(gr[5], x2));
+      }
+    }
+    __syncthreads();
+    // inverse step 2 and its twiddles W_m^(-n2 k1), k1 = 2 w + p
+    if (s2) {
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) a[k2] = buf[(w * R2 + k2) * S + q];
+      Dft<R2, true, V>::run(a);
+      const int k1 = 2 * w + p;
+      if (k1 != 0) {
+#pragma unroll
+        for (int n2 = 1; n2 < R2; ++n2) {
+          V tw = tws[n2 * k1];
+          tw.y = -tw.y;
+          a[n2] = cmul(a[n2], tw);
+        }
+      }
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) buf[(w * R2 + n2) * S + q] = a[n2];
+    }
+    __syncthreads();
+    // inverse step 1 over k for n2 = w
+    if (s1) {
+#pragma unroll
+      for (int k = 0; k < HK; ++k) a[k] = buf[(k * R2 + w) * S + q];
+      Dft<HK, true, V>::run(a);
+      if (p == 0) {
+#pragma unroll
+        for (int n1 = 0; n1 < HK; ++n1) A[n1] = a[n1];
+      } else {
+#pragma unroll
+        for (int n1 = 0; n1 < HK; ++n1) {
+          // A + W_R1^(-n1) B
+          const T c = (T)wr.c[n1], sn = (T)wr.s[n1];
+          const V b = a[n1];
+          A[n1] = n1 == 0 ? cadd(A[0], b) : cx<V>(A[n1].x + b.x * c - b.y * sn, A[n1].y + b.y * c + b.x * sn);
+        }
+      }
+    }
+    if (p == 0) __syncthreads();  // the rows are read before the odd half overwrites them
+  }
+  if (s1) {
+    const int ga = pt / lp.ptb, gb = pt % lp.ptb;
+    const int de = (int)lp.de;  // checked at launch
+    V* __restrict__ obase =
+        reinterpret_cast<V*>(lp.dst) + (long long)lo * lp.lo_dst + ga * lp.da + gb * lp.db + q + w * de;
+    if ((HK - 1) * R2 + w < lp.nout) {
+#pragma unroll
+      for (int n1 = 0; n1 < HK; ++n1) obase[n1 * (R2 * de)] = A[n1];
+    } else {
+#pragma unroll
+      for (int n1 = 0; n1 < HK; ++n1)
+        if (w + R2 * n1 < lp.nout) obase[n1 * (R2 * de)] = A[n1];
+    }
+  }
+}
+
 constexpr int kVPT = 8;
 constexpr int kTile = 4096;  // complex values per workgroup tile
 // FFT kernel selection (set_fft_path): every launcher reads them
 extern std::atomic<bool> g_force_generic;
 extern std::atomic<bool> g_xconv_off;
+// split x pass: 0 where its radix pair is balanced (R2 = R1 / 2), 1 on every
+// eligible length, 2 nowhere (the one-pass kernel k_xconv throughout)
+extern std::atomic<int> g_xconv_split;
+template <int R1, int R2>
+constexpr bool xsplit_balanced() {
+  return 2 * R2 == R1;
+}
 
 template <typename V, int MODE>
 void launch_mode(const LinePass& lp, dim3 grid, int nthr, size_t lds, hipStream_t s) {
@@ -1180,6 +1348,18 @@ void launch_reg2_mode(const LinePass& lp, dim3 grid, int nthr, size_t lds, hipSt
                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
   });
   hipLaunchKernelGGL((k_reg2<V, R1, R2, MODE, LB>), grid, dim3(nthr), lds, s, lp);
+}
+
+template <typename V, int R1, int R2, int LB>
+void launch_xconv_split(const LinePass& lp, dim3 grid, hipStream_t s) {
+  static std::once_flag attr_once;
+  std::call_once(attr_once, [] {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_xconv_split<V, R1, R2, LB>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  });
+  constexpr int S = (3 * LB) | 1;
+  const size_t lds = ((size_t)(R1 / 2) * R2 * S + R1 * R2) * sizeof(V);
+  hipLaunchKernelGGL((k_xconv_split<V, R1, R2, LB>), grid, dim3(xsplit_threads<R1, R2, LB>()), lds, s, lp);
 }
 
 template <typename V, int R1, int R2, int LB>
@@ -1255,6 +1435,11 @@ void launch_reg2_fixed(LinePass lp, hipStream_t s) {
       constexpr int S = Q | 1;
       const size_t lds_x = ((size_t)M * S + M) * sizeof(V);
       const dim3 grid_x(((lp.npts + 7) / 8) * 8 * nlo, 1);
+      const int split = g_xconv_split.load();
+      if (split == 1 || (split == 0 && xsplit_balanced<R1, R2>())) {
+        launch_xconv_split<V, R1, R2, LB>(lp, grid_x, s);
+        return;
+      }
       launch_xconv<V, R1, R2, LB>(lp, grid_x, Q * RMAX, lds_x, s);
       return;
     }

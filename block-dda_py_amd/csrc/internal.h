@@ -99,8 +99,11 @@ struct LinePass {
 template <typename V>
 void launch_line_pass(const LinePass& lp, hipStream_t s);
 // FFT kernel selection (tests compare the paths): 0 register kernels where
-// a length has one (the dedicated x-pass kernel on the 8-column tile), 1 the
-// runtime-radix kernel only, 2 register kernels without the dedicated x pass.
+// a length has one (the dedicated x-pass kernels on the compiled tiles: the
+// split-spectrum one where R2 = R1 / 2, the one-pass one elsewhere), 1 the
+// runtime-radix kernel only, 2 register kernels without the dedicated x pass,
+// 3 the split x pass on every eligible length, 4 the one-pass x pass
+// everywhere.
 void set_fft_path(int path);
 template <typename V>
 void launch_green_build(V* out, const AxisPlan* ax, double lf, double k, hipStream_t s);

@@ -8,6 +8,7 @@ namespace bdda {
 
 std::atomic<bool> g_force_generic{false};
 std::atomic<bool> g_xconv_off{false};
+std::atomic<int> g_xconv_split{0};
 
 BDDA_REG2_LENGTHS(BDDA_REG2_EXTERN)
 
@@ -78,6 +79,7 @@ void launch_line_pass(const LinePass& lp0, hipStream_t s) {
 void set_fft_path(int path) {
   g_force_generic = path == 1;
   g_xconv_off = path == 2;
+  g_xconv_split = path == 3 ? 1 : (path == 4 ? 2 : 0);
 }
 
 template void launch_line_pass<double2>(const LinePass&, hipStream_t);

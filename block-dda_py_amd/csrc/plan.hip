@@ -1377,8 +1377,9 @@ int bdda_set_operator_groups(int groups) {
 }
 
 int bdda_set_fft_path(int path) {
-  if (path < 0 || path > 2) {
-    set_error("bdda_set_fft_path: path must be 0 (auto), 1 (runtime-radix only) or 2 (no dedicated x pass)");
+  if (path < 0 || path > 4) {
+    set_error("bdda_set_fft_path: path must be 0 (auto), 1 (runtime-radix only), 2 (no dedicated x pass), 3 "
+              "(the split-spectrum x pass on every eligible length) or 4 (the one-pass x pass everywhere)");
     return BDDA_E_ARG;
   }
   set_fft_path(path);

@@ -50,8 +50,12 @@ int bdda_version(void);
  * one launch) where R1 is even on the compiled 8- and 4-column tiles, and
  * the runtime-radix LDS Stockham kernels for other lengths; 1 forces the
  * runtime-radix kernels everywhere; 2 is 0 without the dedicated x-pass
- * kernel (the generic register kernel runs the x pass).  1 and 2 exist to
- * cross-check the kernels. */
+ * kernel (the generic register kernel runs the x pass).  Of the dedicated
+ * x pass, 0 runs the split-spectrum kernel (two halves of the spectrum by
+ * the parity of the first digit, half the LDS) where the radix pair is
+ * balanced (R2 = R1 / 2: lengths 32, 50, 128) and the one-pass kernel
+ * elsewhere; 3 runs the split kernel on every eligible length and 4 the
+ * one-pass kernel everywhere.  1 to 4 exist to cross-check the kernels. */
 int bdda_set_fft_path(int path);
 
 /* Column groups of the occupied operator, for the whole process: 1 (default)

@@ -99,7 +99,9 @@ def test_x_pass_kernel_on_the_compiled_tile(n, L, lb):
     one (2N, or 48, 80, 90, 144, 240 where the smallest 5-smooth length
     >= 2N - 1 would be 45, 75, 81, 135, 225: x = 23, 38, 41, 68, 113 here).
     It must agree with the generic register kernel (path 2), the
-    runtime-radix kernels (path 1) and the oracle, and repeat bit for bit."""
+    runtime-radix kernels (path 1), the split-spectrum x pass (path 3) and
+    the one-pass x pass (path 4) on every length, and the oracle, and repeat
+    bit for bit."""
     say(f"x pass kernel n={n} L={L}")
     from bdda_amd import _lib
     from bdda_amd.plan import Plan
@@ -116,17 +118,20 @@ def test_x_pass_kernel_on_the_compiled_tile(n, L, lb):
     At = pm.fft_init(n, 3, lf, k)
     out = {}
     try:
-        for path in (0, 1, 2):
+        for path in (0, 1, 2, 3, 4):
             _lib.check(lib.bdda_set_fft_path(path))
             out[path] = (pm.fft_mvp(n, 3, At, P), pm.fft_mvp(n, 3, At, P, precision="mixed"))
         _lib.check(lib.bdda_set_fft_path(0))
         again = (pm.fft_mvp(n, 3, At, P), pm.fft_mvp(n, 3, At, P, precision="mixed"))
     finally:
         lib.bdda_set_fft_path(0)
-    for path in (0, 1, 2):
+    for path in (0, 1, 2, 3, 4):
         assert rel(out[path][0], ref) < 1e-12, path
         assert rel(out[path][1], ref) < 2e-6, path
     assert rel(out[0][0], out[2][0]) < 1e-13
+    # the split-spectrum x pass (path 3) and the one-pass one (path 4) on every length
+    for path in (3, 4):
+        assert rel(out[path][0], out[2][0]) < 1e-13 and rel(out[path][1], out[2][1]) < 1e-6, path
     # the dedicated kernel repeats bit for bit
     assert np.array_equal(out[0][0], again[0]) and np.array_equal(out[0][1], again[1])
 
