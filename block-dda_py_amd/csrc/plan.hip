@@ -124,7 +124,7 @@ struct bdda_plan {
   unsigned long long lines_version = 0;
   // solver workspace (blocks of 3N x L)
   bdda::DevBuf blk[12];
-  bdda::DevBuf gram_part, coef, dots_d, dots_n, obs_part, Bj64;
+  bdda::DevBuf gram_part, gram_out, coef, dots_d, dots_n, obs_part, Bj64;
   bdda::HostBuf h_gram, h_coef, h_dots;
   // device L x L algebra: coefficient slots, and err / status / scalars /
   // pivots, with the pinned host copy of err and status read once an
@@ -502,18 +502,18 @@ static int gram_launch(Ctx& c, const void* A, const void* B, bool conj, bool f32
   const int np = gram_parts(L, c.rows);
   const size_t one = (size_t)L * L * sizeof(double2);
   int rc = p->gram_part.ensure((size_t)np * ntile * ntile * 4096 * sizeof(double2));
+  if (!rc) rc = p->gram_out.ensure(kGramSlots * one);
   if (!rc) rc = p->h_gram.ensure(kGramSlots * one);
   if (rc) return rc;
-  // the reduction writes the L x L result straight into its pinned host slot
-  // (read after the stream synchronisation), with no copy kernel after it
-  double2* hout = reinterpret_cast<double2*>(p->h_gram.as<char>() + slot * one);
+  double2* dout = p->gram_out.as<double2>() + (size_t)slot * L * L;
   if (f32)
-    launch_gram<float2>((const float2*)A, (const float2*)B, c.rows, L, conj, p->gram_part.as<double2>(), np, hout,
+    launch_gram<float2>((const float2*)A, (const float2*)B, c.rows, L, conj, p->gram_part.as<double2>(), np, dout,
                         c.s);
   else
     launch_gram<double2>((const double2*)A, (const double2*)B, c.rows, L, conj, p->gram_part.as<double2>(), np,
-                         hout, c.s);
+                         dout, c.s);
   BDDA_HIP(hipGetLastError());
+  BDDA_HIP(hipMemcpyAsync(p->h_gram.as<char>() + slot * one, dout, one, hipMemcpyDeviceToHost, c.s));
   return 0;
 }
 
@@ -637,12 +637,14 @@ static int update_gram_launch(Ctx& c, double2* Y, const double2* X, zc a, const 
   const double2* dC = nullptr;
   int rc = coef_upload(c, C, cslot, &dC);
   if (!rc) rc = p->gram_part.ensure((size_t)update_gram_parts(c.rows) * 32 * 32 * sizeof(double2));
+  if (!rc) rc = p->gram_out.ensure(kGramSlots * one);
   if (!rc) rc = p->h_gram.ensure(kGramSlots * one);
   if (rc) return rc;
-  double2* hout = reinterpret_cast<double2*>(p->h_gram.as<char>() + gslot * one);
+  double2* dout = p->gram_out.as<double2>() + (size_t)gslot * L * L;
   launch_update_gram(Y, X, make_double2(a.real(), a.imag()), W1, nullptr, make_double2(0.0, 0.0), dC, nullptr,
-                     make_double2(0.0, 0.0), c.rows, L, conj ? 1 : 2, p->gram_part.as<double2>(), hout, c.s);
+                     make_double2(0.0, 0.0), c.rows, L, conj ? 1 : 2, p->gram_part.as<double2>(), dout, c.s);
   BDDA_HIP(hipGetLastError());
+  BDDA_HIP(hipMemcpyAsync(p->h_gram.as<char>() + gslot * one, dout, one, hipMemcpyDeviceToHost, c.s));
   return 0;
 }
 
