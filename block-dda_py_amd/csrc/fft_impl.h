@@ -89,37 +89,6 @@ template <typename V> __device__ __forceinline__ V cmul(V a, V b) {
 template <typename V> __device__ __forceinline__ V cscale(V a, typename RealOf<V>::T s) {
   return cx<V>(a.x * s, a.y * s);
 }
-__device__ __forceinline__ double fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
-__device__ __forceinline__ float fmad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-// a0 x0 + a1 x1 + a2 x2 as one fused multiply-add chain per part (a
-// multiply and five FMAs, where three complex products and their sums
-// would take eight operations): a row of the 3x3 Green multiply.
-template <typename V>
-__device__ __forceinline__ V cdot3(V a0, V x0, V a1, V x1, V a2, V x2) {
-  using T = typename RealOf<V>::T;
-  T re = a0.x * x0.x, im = a0.x * x0.y;
-  re = fmad(-a0.y, x0.y, re);
-  im = fmad(a0.y, x0.x, im);
-  re = fmad(a1.x, x1.x, re);
-  im = fmad(a1.x, x1.y, im);
-  re = fmad(-a1.y, x1.y, re);
-  im = fmad(a1.y, x1.x, im);
-  re = fmad(a2.x, x2.x, re);
-  im = fmad(a2.x, x2.y, im);
-  re = fmad(-a2.y, x2.y, re);
-  im = fmad(a2.y, x2.x, im);
-  return cx<V>(re, im);
-}
-// x <- G x for the symmetric 3x3 Green block g = (xx, xy, xz, yy, yz, zz)
-template <typename V>
-__device__ __forceinline__ void green_apply(const V* g, V& x0, V& x1, V& x2) {
-  const V y0 = cdot3(g[0], x0, g[1], x1, g[2], x2);
-  const V y1 = cdot3(g[1], x0, g[3], x1, g[4], x2);
-  const V y2 = cdot3(g[2], x0, g[4], x1, g[5], x2);
-  x0 = y0;
-  x1 = y1;
-  x2 = y2;
-}
 // a * (-i) for the forward transform, a * (+i) for the inverse one
 template <bool INV, typename V> __device__ __forceinline__ V rotq(V a) {
   return INV ? cx<V>(-a.y, a.x) : cx<V>(a.y, -a.x);
@@ -456,14 +425,14 @@ __device__ void green_multiply(V* buf, const LinePass& lp, int pt0, int np_here,
     if (ay > lp.my - ay) { ay = lp.my - ay; sy = -1; }
     if (az > lp.mz - az) { az = lp.mz - az; sz = -1; }
     const V* g = G + (((long long)ay * lp.hz + az) * lp.hx + ax) * 6;
-    const V gs[6] = {g[0], cscale(g[1], sx * sy), cscale(g[2], sx * sz), g[3], cscale(g[4], sy * sz), g[5]};
+    const V gxx = g[0], gyy = g[3], gzz = g[5];
+    const V gxy = cscale(g[1], sx * sy), gxz = cscale(g[2], sx * sz), gyz = cscale(g[4], sy * sz);
     V* row = buf + qx * QQ + p * Q;
     for (int li = 0; li < Lb; ++li) {
-      V x0 = row[li], x1 = row[Lb + li], x2 = row[2 * Lb + li];
-      green_apply(gs, x0, x1, x2);
-      row[li] = x0;
-      row[Lb + li] = x1;
-      row[2 * Lb + li] = x2;
+      const V x0 = row[li], x1 = row[Lb + li], x2 = row[2 * Lb + li];
+      row[li] = cadd(cadd(cmul(gxx, x0), cmul(gxy, x1)), cmul(gxz, x2));
+      row[Lb + li] = cadd(cadd(cmul(gxy, x0), cmul(gyy, x1)), cmul(gyz, x2));
+      row[2 * Lb + li] = cadd(cadd(cmul(gxz, x0), cmul(gyz, x1)), cmul(gzz, x2));
     }
   }
   __syncthreads();
@@ -802,11 +771,10 @@ __attribute__((amdgpu_waves_per_eu(reg2_waves_per_simd<V, R2, MODE>()))) k_reg2(
         const int li = t % Lb, kx = (t / Lb) % M, pp = t / (Lb * M);
         const V* g = gls + (pp * M + kx) * 6;
         V* row = buf + kx * QQ + pp * Q;
-        V x0 = row[li], x1 = row[Lb + li], x2 = row[2 * Lb + li];
-        green_apply(g, x0, x1, x2);
-        row[li] = x0;
-        row[Lb + li] = x1;
-        row[2 * Lb + li] = x2;
+        const V x0 = row[li], x1 = row[Lb + li], x2 = row[2 * Lb + li];
+        row[li] = cadd(cadd(cmul(g[0], x0), cmul(g[1], x1)), cmul(g[2], x2));
+        row[Lb + li] = cadd(cadd(cmul(g[1], x0), cmul(g[3], x1)), cmul(g[4], x2));
+        row[2 * Lb + li] = cadd(cadd(cmul(g[2], x0), cmul(g[4], x1)), cmul(g[5], x2));
       }
       __syncthreads();
       if (s1) {
@@ -1104,11 +1072,10 @@ __device__ __forceinline__ void xconv_point(const LinePass& lp, V* buf, const V*
     V* row = buf + r * S;
 #pragma unroll
     for (int li = grp; li < LB; li += G) {
-      V x0 = row[li], x1 = row[LB + li], x2 = row[2 * LB + li];
-      green_apply(gr, x0, x1, x2);
-      row[li] = x0;
-      row[LB + li] = x1;
-      row[2 * LB + li] = x2;
+      const V x0 = row[li], x1 = row[LB + li], x2 = row[2 * LB + li];
+      row[li] = cadd(cadd(cmul(gr[0], x0), cmul(gr[1], x1)), cmul(gr[2], x2));
+      row[LB + li] = cadd(cadd(cmul(gr[1], x0), cmul(gr[3], x1)), cmul(gr[4], x2));
+      row[2 * LB + li] = cadd(cadd(cmul(gr[2], x0), cmul(gr[4], x1)), cmul(gr[5], x2));
     }
   }
   __syncthreads();
@@ -1289,11 +1256,10 @@ __attribute__((amdgpu_waves_per_eu(xsplit_waves_per_simd<V, R2>()))) k_xconv_spl
       V* row = buf + r * S;
 #pragma unroll
       for (int li = grp; li < LB; li += G) {
-        V x0 = row[li], x1 = row[LB + li], x2 = row[2 * LB + li];
-        green_apply(gr, x0, x1, x2);
-        row[li] = x0;
-        row[LB + li] = x1;
-        row[2 * LB + li] = x2;
+        const V x0 = row[li], x1 = row[LB + li], x2 = row[2 * LB + li];
+        row[li] = cadd(cadd(cmul(gr[0], x0), cmul(gr[1], x1)), cmul(gr[2], x2));
+        row[LB + li] = cadd(cadd(cmul(gr[1], x0), cmul(gr[3], x1)), cmul(gr[4], x2));
+        row[2 * LB + li] = cadd(cadd(cmul(gr[2], x0), cmul(gr[4], x1)), cmul(gr[5], x2));
       }
     }
     __syncthreads();
