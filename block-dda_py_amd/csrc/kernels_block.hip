@@ -253,8 +253,20 @@ __global__ void __launch_bounds__(256) k_gram_reduce(const double2* __restrict__
   const int j = (tile / ntile) * TL + w / TL, l = (tile % ntile) * TL + w % TL;
   const bool live = id < (long long)ntile2 * te && j < L && l < L;
   double2 s = make_double2(0.0, 0.0);
-  if (live)
-    for (int p = g; p < nparts; p += G) s = zadd(s, partial[((long long)p * ntile2 + tile) * te + w]);
+  if (live) {
+    // eight parts' loads in flight at once; the sum keeps part order
+    for (int p0 = g; p0 < nparts; p0 += 8 * G) {
+      double2 v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int p = p0 + u * G;
+        v[u] = p < nparts ? partial[((long long)p * ntile2 + tile) * te + w] : make_double2(0.0, 0.0);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (p0 + u * G < nparts) s = zadd(s, v[u]);
+    }
+  }
   sh[g * E + e] = s;
   __syncthreads();
   if (g == 0 && live) {
