@@ -1459,6 +1459,11 @@ void launch_reg2(LinePass lp, hipStream_t s) {
   if (lp.Q == 3 * lp.Lb && lp.Lb == 4) return launch_reg2_fixed<V, R1, R2, 4>(lp, s);
   const bool conv = lp.mode == PASS_CONV_X;
   int P = std::min(512 / (lp.Q * RMAX), kTile / (M * lp.Q));
+  // the x pass also stages each point's Green lines (M x 6 entries): on
+  // narrow tiles (Q = 3) they outweigh the line tile, so the points per
+  // block are also held to what fits the CU's LDS
+  constexpr size_t kLdsBytes = 160 * 1024;
+  if (conv) P = std::min<long long>(P, (long long)(kLdsBytes / sizeof(V) - M) / ((long long)M * (lp.Q + 6)));
   if (P < 1) P = 1;
   if (P > lp.npts) P = lp.npts;
   lp.P = P;
@@ -1469,6 +1474,10 @@ void launch_reg2(LinePass lp, hipStream_t s) {
   }
   const int nlo = lp.nlo > 0 ? lp.nlo : 1;
   const size_t lds = ((size_t)M * P * lp.Q + M + (conv ? (size_t)P * M * 6 : 0)) * sizeof(V);
+  if (lds > kLdsBytes) {
+    set_error("register FFT launch exceeds the LDS");
+    return;
+  }
   const int nblk = (lp.npts + P - 1) / P;
   dim3 grid(nblk, nlo);
   if (conv) grid = dim3(((nblk + 7) / 8) * 8 * nlo, 1);

@@ -206,6 +206,70 @@ def test_block_mvp_on_a_confined_target_matches_oracle(n, L, box):
     assert pl.occupied_extent == box
 
 
+@pytest.mark.parametrize("draw", range(12))
+def test_block_mvp_random_configurations_match_oracle(draw):
+    """Seeded random configurations of the occupied-block operator: cuboid
+    axes of 2-48 cells (any length the plan picks: register lengths, the
+    runtime-radix kernels, balanced and unbalanced x radix pairs), L of
+    1-70 (compiled 8- and 4-column tiles and runtime tiles), and the target
+    a random fraction of a random sub-box anchored at the origin (so the
+    passes prune to its extent) or scattered over the whole cuboid.
+    Against the oracle's _block_mvp at 1e-12, and the off-diagonal cuboid
+    product in complex64 mode at 2e-6."""
+    from bl_krylov import bl_krylov as pk
+    from mvp_fft import mvp_fft as pm
+    rng = np.random.default_rng(1000 + draw)
+    while True:
+        n = tuple(int(v) for v in rng.integers(2, 49, size=3))
+        if np.prod(n) <= 24000:
+            break
+    L = int(rng.integers(1, 71))
+    confined = bool(rng.integers(0, 2))
+    box = tuple(int(rng.integers(1, v + 1)) for v in n) if confined else n
+    say(f"random block mvp draw={draw} n={n} L={L} box={box}")
+    lf, k = 0.03, float(rng.uniform(5.0, 12.0))
+    cells = np.array([(ix * n[1] + iy) * n[2] + iz for ix in range(box[0]) for iy in range(box[1])
+                      for iz in range(box[2])])
+    frac = float(rng.uniform(0.05, 1.0))
+    address = np.unique(rng.choice(cells, size=max(1, int(frac * cells.size)), replace=False)).astype(np.int64)
+    N = address.size
+    diag = rng.standard_normal(3 * N) + 1j * rng.standard_normal(3 * N) + 3.0
+    jpre = 1.0 / diag
+    S = rng.standard_normal((3 * N, L)) + 1j * rng.standard_normal((3 * N, L))
+    idx = pk._build_scatter_index(address, 3)
+    a = pk._block_mvp(n, 3, pm.fft_init(n, 3, lf, k), idx, diag, S, jpre)
+    b = okr.block_mvp(n, 3, omvp.fft_init(n, 3, lf, k), idx, diag, S, jpre)
+    assert rel(a, b) < (1e-12 if N > 1 else 5e-12)
+    ncell = int(np.prod(n))
+    P = rng.standard_normal((3 * ncell, L)) + 1j * rng.standard_normal((3 * ncell, L))
+    ref = omvp.fft_mvp(n, 3, omvp.fft_init(n, 3, lf, k), P)
+    assert rel(pm.fft_mvp(n, 3, pm.fft_init(n, 3, lf, k), P, precision="mixed"), ref) < 2e-6
+
+
+@pytest.mark.parametrize("n,L", [((36, 4, 30), 23), ((64, 5, 4), 17), ((64, 5, 4), 37), ((3, 4, 96), 29)])
+def test_prime_block_width_runs_one_column_tiles(n, L):
+    """A prime L above the tile cap leaves a 1-column tile (Q = 3), whose
+    x pass stages more Green entries than line data; the launcher must keep
+    it inside the CU's LDS (it once asked 183 KB on (36, 4, 30), L = 23).
+    Against the oracle's _block_mvp, whole cuboid occupied."""
+    say(f"prime block width n={n} L={L}")
+    from bl_krylov import bl_krylov as pk
+    from mvp_fft import mvp_fft as pm
+    from bdda_amd.plan import Plan
+    assert Plan(n).column_tile(L) == 1
+    lf, k = 0.03, 9.5
+    rng = np.random.default_rng(29)
+    ncell = int(np.prod(n))
+    address = np.arange(ncell, dtype=np.int64)
+    diag = rng.standard_normal(3 * ncell) + 1j * rng.standard_normal(3 * ncell) + 3.0
+    jpre = 1.0 / diag
+    S = rng.standard_normal((3 * ncell, L)) + 1j * rng.standard_normal((3 * ncell, L))
+    idx = pk._build_scatter_index(address, 3)
+    a = pk._block_mvp(n, 3, pm.fft_init(n, 3, lf, k), idx, diag, S, jpre)
+    b = okr.block_mvp(n, 3, omvp.fft_init(n, 3, lf, k), idx, diag, S, jpre)
+    assert rel(a, b) < 1e-12
+
+
 def test_block_mvp_after_other_products_on_the_same_plan():
     """The occupied operator keeps zeros on T1's unoccupied z lines and
     skips them; cuboid products (every line written), the complex64
