@@ -104,3 +104,35 @@ def test_column_tile_out_of_range_is_refused(small_batch):
         dd._plan.set_column_tile(17)
     with pytest.raises(ValueError, match="tile"):
         dd._plan.set_column_tile(-1)
+
+
+@pytest.mark.parametrize("draw", range(3))
+def test_random_batches_match_single_solves(draw):
+    """Seeded random batches: six particles of 6-14 cells, each with its own
+    L of 1-40 (prime widths above the tile cap take 1-column tiles), a
+    random precision, L x L path and stream count.  Each instance must come
+    out of the batch bit-identical to its one-at-a-time solve."""
+    from bdda_amd import _lib, workloads
+    from bdda_amd.batch import solve_batch
+    rng = np.random.default_rng(3000 + draw)
+    precision, la_path = str(rng.choice(["fp64", "mixed"])), int(rng.integers(0, 2))
+    nstreams = int(rng.integers(1, 7))
+    wl = dict(workloads.BATCH_32, grid=(6, 14), itermax=300)
+    insts = workloads.batch_instances(wl, count=12)
+    pick = rng.choice(len(insts), size=6, replace=False)
+    insts = [dict(insts[i], L=int(rng.integers(1, 41))) for i in pick]
+    say(f"random batch draw={draw} L={[w['L'] for w in insts]} {precision} la={la_path} streams={nstreams}")
+    lib = _lib.load()
+    _lib.check(lib.bdda_set_la_path(la_path))
+    try:
+        dds = [workloads.make_dipoles(w, geometry=workloads.build_geometry(w), precision=precision) for w in insts]
+        ref = []
+        for dd in dds:
+            dd.solve_matrix_equation()
+            ref.append((dd.X_device.clone(), dd.iter_fin, list(dd.err_history), dd.converge))
+        solve_batch(dds, nstreams=nstreams)
+    finally:
+        lib.bdda_set_la_path(0)
+    for dd, (X, it, hist, conv) in zip(dds, ref):
+        assert torch.equal(dd.X_device, X)
+        assert dd.iter_fin == it and dd.err_history == hist and dd.converge == conv

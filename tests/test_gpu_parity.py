@@ -400,6 +400,58 @@ def test_wide_blocks_match_oracle(solver, L):
     assert np.max(np.abs(sb - ref["S_bk"]) / np.abs(ref["S_bk"])) < 1e-6
 
 
+@pytest.mark.parametrize("draw", range(10))
+def test_random_particles_solve_as_the_oracle(draw):
+    """Seeded random solves: GRE particles of 300-700 dipoles (size, ratios,
+    beta, seed drawn), isotropic or birefringent m_p, L of 1-40 (prime
+    widths above the tile cap take 1-column tiles), either solver (block-
+    COCG-rq only on isotropic m_p, whose preconditioned operator is complex
+    symmetric, its condition in the reference), the host or the device
+    L x L algebra, float64 or mixed.  Where the oracle's float64
+    solve converges to 1e-8 the GPU solve must as well, with the same
+    solution to 1e-6 and S, C within 1e-6 (float64), or S within 1e-4 at a
+    true residual of 1e-5 (mixed).  Where the oracle does not converge,
+    only the GPU's own report is checked."""
+    from bdda_amd import _lib
+    rng = np.random.default_rng(2000 + draw)
+    r = float(rng.uniform(0.05, 0.10))
+    bc, ab = float(rng.uniform(0.6, 1.6)), float(rng.uniform(0.6, 1.4))
+    beta, seed = float(rng.choice([0.0, 0.1])), int(rng.integers(0, 1000))
+    iso = bool(rng.integers(0, 2))
+    if iso:
+        m_p = (complex(rng.uniform(1.3, 1.7), 10 ** rng.uniform(-4, -1.5)),) * 3
+    else:
+        m_p = tuple(complex(rng.uniform(1.3, 1.7), 10 ** rng.uniform(-4, -1.5)) for _ in range(3))
+    L, mm = int(rng.integers(1, 41)), float(rng.uniform(1.0, 1.33))
+    # block-COCG-rq needs the complex-symmetric operator of an isotropic m_p
+    solver = str(rng.choice(["cocg_rq", "bicgstab"])) if iso else "bicgstab"
+    la, precision = int(rng.integers(0, 2)), str(rng.choice(["fp64", "mixed"]))
+    say(f"random solve draw={draw} L={L} {solver} la={la} {precision}")
+    prob = _problem(r=r, bc=bc, ab=ab, beta=beta, m_p=m_p, L=L, mm=mm, seed=seed)
+    ref = _oracle_solve(prob, "cocg" if solver == "cocg_rq" else "bicgstab", 1e-8, 300)
+    lib = _lib.load()
+    try:
+        _lib.check(lib.bdda_set_la_path(la))
+        dd = _gpu_dd(prob, solver, 1e-8 if precision == "fp64" else 1e-5, 300, precision)
+    finally:
+        lib.bdda_set_la_path(0)
+    assert dd.converge == (dd.err_fin < dd.tol)
+    if not ref["err_fin"] < 1e-8:
+        return
+    assert dd.converge
+    st, sp = dd.compute_PCAS_observable_S_fw()
+    sb = dd.compute_OCBS_observable_S_bk()
+    if precision == "fp64":
+        assert rel(dd.X, ref["X"]) < 1e-6 and dd.true_residual < 1e-7
+        for a, b in [(st, ref["S_fw_theta"]), (sp, ref["S_fw_phi"]), (sb, ref["S_bk"])]:
+            assert np.max(np.abs(a - b) / np.abs(b)) < 1e-6
+        np.testing.assert_allclose(dd.compute_C_ext(), ref["C_ext"], rtol=1e-6)
+    else:
+        assert dd.true_residual <= 1e-5
+        for a, b in [(st, ref["S_fw_theta"]), (sp, ref["S_fw_phi"]), (sb, ref["S_bk"])]:
+            assert np.max(np.abs(a - b) / np.abs(b)) < 1e-4
+
+
 def test_block_width_limit_is_reported():
     from bdda_amd.plan import Plan
     p = Plan((4, 4, 4))
