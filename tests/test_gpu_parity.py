@@ -246,6 +246,68 @@ def test_block_mvp_random_configurations_match_oracle(draw):
     assert rel(pm.fft_mvp(n, 3, pm.fft_init(n, 3, lf, k), P, precision="mixed"), ref) < 2e-6
 
 
+@pytest.mark.parametrize("draw", range(4))
+def test_random_sequences_on_one_plan_match_oracle(draw):
+    """One plan through a seeded random sequence of twelve steps: new
+    targets (random sub-boxes at the origin or scattered cells), occupied
+    block products and cuboid products (both precisions) of random widths,
+    a Green spectrum rebuilt at another wavenumber, and column tiles forced
+    or left automatic.  Every product must equal the oracle's for the
+    plan's current state, so no cached line list, extent, zeroed scratch,
+    workspace size or spectrum may outlive what it was built for."""
+    from bdda_amd.plan import Plan
+    rng = np.random.default_rng(4000 + draw)
+    while True:
+        n = tuple(int(v) for v in rng.integers(3, 21, size=3))
+        if np.prod(n) <= 6000:
+            break
+    ncell = int(np.prod(n))
+    lf, k = 0.03, float(rng.uniform(5.0, 12.0))
+    p = Plan(n)
+    p.green_init(lf, k)
+    Ao = omvp.fft_init(n, 3, lf, k)
+    tgt = None
+    ops = []
+    for step in range(12):
+        op = str(rng.choice(["target", "block", "block", "cuboid", "green", "tile"])) if tgt else "target"
+        if op == "target":
+            box = tuple(int(rng.integers(1, v + 1)) for v in n) if rng.integers(0, 2) else n
+            cells = np.array([(ix * n[1] + iy) * n[2] + iz for ix in range(box[0]) for iy in range(box[1])
+                              for iz in range(box[2])])
+            address = np.unique(rng.choice(cells, size=max(2, int(rng.uniform(0.1, 1.0) * cells.size)))).astype(
+                np.int64)
+            N = address.size
+            diag = rng.standard_normal(3 * N) + 1j * rng.standard_normal(3 * N) + 3.0
+            tgt = (address, diag, 1.0 / diag)
+            p.set_target(*tgt)
+            ops.append(f"target N={N} box={box}")
+        elif op == "block":
+            L = int(rng.integers(1, 41))
+            address, diag, jpre = tgt
+            S = rng.standard_normal((3 * address.size, L)) + 1j * rng.standard_normal((3 * address.size, L))
+            idx = okr.scatter_index(address, 3)
+            got = p.block_mvp(torch.tensor(S, device="cuda")).cpu().numpy()
+            ref = okr.block_mvp(n, 3, Ao, idx, diag, S, jpre)
+            ops.append(f"block L={L}")
+            assert rel(got, ref) < (1e-12 if address.size > 1 else 5e-12), ops
+        elif op == "cuboid":
+            L, prec = int(rng.integers(1, 41)), str(rng.choice(["fp64", "mixed"]))
+            P = rng.standard_normal((3 * ncell, L)) + 1j * rng.standard_normal((3 * ncell, L))
+            got = p.mvp_offdiag(torch.tensor(P, device="cuda"), prec).cpu().numpy()
+            ops.append(f"cuboid L={L} {prec}")
+            assert rel(got, omvp.fft_mvp(n, 3, Ao, P)) < (1e-12 if prec == "fp64" else 2e-6), ops
+        elif op == "green":
+            k = float(rng.uniform(5.0, 12.0))
+            p.green_init(lf, k)
+            Ao = omvp.fft_init(n, 3, lf, k)
+            ops.append(f"green k={k:.3f}")
+        else:
+            lb = int(rng.choice([0, 1, 2, 3, 4, 8]))
+            p.set_column_tile(lb)
+            ops.append(f"tile {lb}")
+    say(f"plan sequence n={n}: " + "; ".join(ops))
+
+
 @pytest.mark.parametrize("n,L", [((36, 4, 30), 23), ((64, 5, 4), 17), ((64, 5, 4), 37), ((3, 4, 96), 29)])
 def test_prime_block_width_runs_one_column_tiles(n, L):
     """A prime L above the tile cap leaves a 1-column tile (Q = 3), whose
