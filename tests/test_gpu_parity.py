@@ -825,6 +825,27 @@ def test_voxeliser_bit_exact(beta):
     assert np.array_equal(m_gpu, m_ref)
 
 
+@pytest.mark.parametrize("draw", range(6))
+def test_voxeliser_bit_exact_on_random_particles(draw):
+    """Seeded random GRE particles (size, both ratios, beta up to 0.3, shape
+    seed): the GPU nearest distances and inside mask equal the oracle's
+    bit for bit."""
+    from shape_model.gaussian_ellipsoid import gaussian_ellipsoid_shape_model as G
+    rng = np.random.default_rng(5000 + draw)
+    r, bc, ab = float(rng.uniform(0.06, 0.2)), float(rng.uniform(0.5, 1.6)), float(rng.uniform(0.5, 1.4))
+    beta, seed = float(rng.uniform(0.0, 0.3)), int(rng.integers(0, 10000))
+    say(f"voxeliser draw={draw} r={r:.3f} bc={bc:.2f} ab={ab:.2f} beta={beta:.2f} seed={seed}")
+    g = G(r, bc, ab, beta)
+    pts, _ = g.compute_r_points_on_GRE(np.random.default_rng(seed))
+    _, n, grid = g.create_cuboid_lattice_that_encloses_GRE_shape(pts)
+    d_gpu = g.find_nearest_distance_from_the_GRE_surf(grid, pts)
+    d_ref = ogre.nearest_distance(grid, pts)
+    assert np.array_equal(d_gpu, d_ref)
+    m_gpu = g.extract_lattice_address_in_GRE_volume(g.lattice_lf, g.distance_factor, n, d_gpu)
+    m_ref = ogre.inside_mask(g.lattice_lf, g.distance_factor, n, d_ref)
+    assert np.array_equal(m_gpu, m_ref)
+
+
 @pytest.mark.parametrize("groups", [2, 3, 4])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_operator_groups_are_bit_identical(groups, precision):
