@@ -872,6 +872,34 @@ def test_operator_groups_are_bit_identical(groups, precision):
     assert torch.equal(dd.X_device, ref.X_device)
 
 
+@pytest.mark.parametrize("draw", range(4))
+def test_operator_groups_bit_identical_on_random_widths(draw):
+    """Seeded random widths L of 1-90 (compiled and runtime tiles, from one
+    tile to dozens, groups of uneven size or fewer tiles than groups), 2-4
+    groups, either precision: the grouped occupied operator equals the
+    plain one bit for bit, on the block product and on a whole solve."""
+    from bdda_amd import _lib
+    rng = np.random.default_rng(6000 + draw)
+    L, groups = int(rng.integers(1, 91)), int(rng.integers(2, 5))
+    precision = str(rng.choice(["fp64", "mixed"]))
+    say(f"operator groups draw={draw} L={L} groups={groups} {precision}")
+    lib = _lib.load()
+    prob = _problem(r=float(rng.uniform(0.05, 0.09)), L=L, seed=int(rng.integers(0, 1000)))
+    ref = _gpu_dd(prob, "bicgstab", 1e-8, 80, precision=precision)
+    g = torch.Generator(device="cuda").manual_seed(draw)
+    S = torch.randn(ref._B_dev.shape, dtype=torch.complex128, device="cuda", generator=g)
+    AS_ref = ref._plan.block_mvp(S)
+    try:
+        _lib.check(lib.bdda_set_operator_groups(groups))
+        AS = ref._plan.block_mvp(S)
+        dd = _gpu_dd(prob, "bicgstab", 1e-8, 80, precision=precision)
+    finally:
+        lib.bdda_set_operator_groups(1)
+    assert torch.equal(AS, AS_ref)
+    assert dd.iter_fin == ref.iter_fin and dd.err_history == ref.err_history
+    assert torch.equal(dd.X_device, ref.X_device)
+
+
 def test_operator_groups_out_of_range_refused():
     from bdda_amd import _lib
     lib = _lib.load()
