@@ -1922,6 +1922,10 @@ int bdda_solve(bdda_plan* p, int solver, int precision, const void* B_dev, const
       if (!hist.empty()) hist.back() = res;
       if (inner.empty()) break;
     }
+    // a guess that already meets the tolerance needs no cycle: its true
+    // residual is the solve's one history entry (an empty history reports
+    // a solve that never ran, err_fin = inf, as the reference does)
+    if (hist.empty() && res < tol) hist.push_back(res);
     it = total > 0 ? total - 1 : 0;
     err = res;
     flags |= std::min(cycles, 63) << 2;

@@ -694,6 +694,48 @@ def test_warm_start_and_fallback():
     assert rel(dd2.X, dd0.X) < 1e-12
 
 
+@pytest.mark.parametrize("draw", range(4))
+def test_warm_start_on_random_guesses(draw):
+    """Seeded random guesses X0 = X (1 + e Z) around the oracle's solution,
+    e from 1e-6 to 30, on a random particle, width, solver and precision.
+    A guess whose preconditioned residual is below the zero start's is taken
+    and the solve reaches the tolerance from it (in fewer iterations for a
+    close guess); a worse one is dropped, and the solve then takes the
+    zero start's iterations to the same solution."""
+    rng = np.random.default_rng(7000 + draw)
+    L = int(rng.integers(1, 30))
+    solver, precision = str(rng.choice(["cocg_rq", "bicgstab"])), str(rng.choice(["fp64", "mixed"]))
+    prob = _problem(r=float(rng.uniform(0.05, 0.09)), L=L, seed=int(rng.integers(0, 1000)))
+    ref = _oracle_solve(prob, "cocg" if solver == "cocg_rq" else "bicgstab", 1e-10, 200)
+    dd0 = _gpu_dd(prob, solver, 1e-5, 200, precision)
+    for e in (1e-6, 1e-3, 0.1, 3.0, 30.0):
+        Z = rng.standard_normal(ref["X"].shape) + 1j * rng.standard_normal(ref["X"].shape)
+        X0 = ref["X"] * (1 + e * Z)
+        say(f"warm start draw={draw} L={L} {solver} {precision} e={e}")
+        dd = _gpu_dd(prob, solver, 1e-5, 200, precision, X0=X0)
+        assert dd.converge and dd.true_residual <= 1e-5
+        if dd.warm_start_used:
+            if e <= 1e-3:
+                assert dd.iter_fin < dd0.iter_fin
+        else:
+            assert dd.iter_fin == dd0.iter_fin and rel(dd.X, dd0.X) < 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_warm_start_that_already_converged(precision):
+    """A guess that already meets the tolerance: the float64 solve runs the
+    reference's one iteration; the mixed solve needs no complex64 cycle and
+    reports the guess's true residual as its one history entry (it once
+    reported no history, so the solve read as not converged)."""
+    say(f"converged warm start {precision}")
+    prob = _problem(L=6)
+    ref = _oracle_solve(prob, "bicgstab", 1e-10, 80)
+    dd = _gpu_dd(prob, "bicgstab", 1e-5, 80, precision, X0=ref["X"])
+    assert dd.warm_start_used and dd.converge and dd.iter_fin == 0
+    assert len(dd.err_history) == 1 and dd.err_fin < 1e-5 and dd.true_residual < 1e-5
+    assert rel(dd.X, ref["X"]) < 1e-6
+
+
 def test_cocg_nearly_dependent_orientations():
     """Two orientations 1e-9 rad apart make the right-hand-side block nearly
     rank deficient (its Gram matrix is singular to float64, so a plain
