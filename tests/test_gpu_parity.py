@@ -721,6 +721,39 @@ def test_warm_start_on_random_guesses(draw):
             assert dd.iter_fin == dd0.iter_fin and rel(dd.X, dd0.X) < 1e-12
 
 
+@pytest.mark.parametrize("draw", range(8))
+def test_solver_limits_on_random_draws(draw):
+    """Seeded random tolerances (1e-1 to 1e-9, above the rounding floor) and
+    iteration budgets (0 to 40) at small L, where the float64 iterates
+    follow the oracle's: the same iteration count, residual history,
+    reported error and convergence flag, including a budget of 0 (no
+    iteration, error inf, not converged) and a budget that runs out.  The
+    mixed solve under the same limits (refinement checks every 1-20
+    iterations or never) must report a consistent state: converged exactly
+    when its error, its true residual, is below the tolerance."""
+    rng = np.random.default_rng(8000 + draw)
+    L = int(rng.integers(1, 9))
+    solver = str(rng.choice(["cocg_rq", "bicgstab"])) if L > 1 else "bicgstab"
+    tol = float(rng.choice([1e-1, 1e-3, 1e-6, 1e-9]))
+    itermax = int(rng.choice([0, 1, 2, 5, 40]))
+    every = int(rng.choice([0, 1, 3, 20]))
+    say(f"solver limits draw={draw} L={L} {solver} tol={tol} itermax={itermax} refine_every={every}")
+    prob = _problem(r=float(rng.uniform(0.05, 0.08)), L=L, seed=int(rng.integers(0, 1000)))
+    ref = _oracle_solve(prob, "cocg" if solver == "cocg_rq" else "bicgstab", tol, itermax)
+    dd = _gpu_dd(prob, solver, tol, itermax)
+    assert dd.iter_fin == ref["iter_fin"] and dd.converge == ref["converge"]
+    if itermax == 0:
+        assert dd.err_fin == np.inf and ref["err_fin"] == np.inf and not dd.converge
+    else:
+        np.testing.assert_allclose(dd.err_history, ref["history"], rtol=1e-6)
+        np.testing.assert_allclose(dd.err_fin, ref["err_fin"], rtol=1e-6)
+    dm = _gpu_dd(prob, solver, tol, itermax, "mixed", refine_every=every)
+    assert dm.converge == (dm.err_fin < tol)
+    if dm.converge:
+        assert dm.true_residual < tol
+    assert len(dm.err_history) <= max(itermax, 1)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_warm_start_that_already_converged(precision):
     """A guess that already meets the tolerance: the float64 solve runs the
