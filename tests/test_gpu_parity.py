@@ -754,6 +754,50 @@ def test_solver_limits_on_random_draws(draw):
     assert len(dm.err_history) <= max(itermax, 1)
 
 
+@pytest.mark.parametrize("draw", range(3))
+def test_guess_projection_is_the_least_squares_fit(draw):
+    """bdda_project_guess replaces X0 by X0 C with C the least-squares fit
+    min ||jpre (B - A X0 C)||_F (normal equations with a relative ridge of
+    1e-13 tr(W^H W) / L, W = jpre A X0).  Against the same fit in NumPy on
+    the oracle's operator: a noisy guess, the exact solution with its
+    columns mixed (the fit must undo the mixing), and a guess with two
+    equal columns (the ridge keeps the normal equations solvable)."""
+    rng = np.random.default_rng(9000 + draw)
+    L = int(rng.integers(3, 20))
+    prob = _problem(r=float(rng.uniform(0.05, 0.08)), L=L, seed=int(rng.integers(0, 1000)))
+    ref = _oracle_solve(prob, "bicgstab", 1e-11, 200)
+    dd = _gpu_dd(prob, "bicgstab", 1e-2, 1)
+    name, n, lf, grid, is_in, m_p, eu, wl, mm = prob
+    address = np.flatnonzero(is_in.ravel()) if is_in.ndim > 1 else np.flatnonzero(is_in)
+    idx = okr.scatter_index(address, 3)
+    diag, B, At = ref["diag"], ref["B"], ref["A_tilde"]
+    jpre = 1.0 / diag
+    Bj = B * jpre[:, None]
+    X = ref["X"]
+    Z = rng.standard_normal(X.shape) + 1j * rng.standard_normal(X.shape)
+    M = rng.standard_normal((L, L)) + 1j * rng.standard_normal((L, L)) + 3 * np.eye(L)
+    dup = X * (1 + 0.1 * Z)
+    dup[:, 1] = dup[:, 0]
+    for kind, X0 in (("noisy", X * (1 + 0.3 * Z)), ("mixed columns", X @ M), ("equal columns", dup)):
+        say(f"guess projection draw={draw} L={L} {kind}")
+        W = okr.block_mvp(n, 3, At, idx, diag, X0, jpre)
+        G = W.conj().T @ W
+        G = G + 1e-13 * np.trace(G).real / L * np.eye(L)
+        C = np.linalg.solve(G, W.conj().T @ Bj)
+        want_before = np.linalg.norm(Bj - W) / np.linalg.norm(Bj)
+        want_after = np.linalg.norm(Bj - W @ C) / np.linalg.norm(Bj)
+        X0d = torch.tensor(X0, dtype=torch.complex128, device="cuda").contiguous()
+        before, after = dd._plan.project_guess(dd._B_dev, X0d)
+        got = X0d.cpu().numpy()
+        assert np.isfinite(got).all() and after <= before
+        np.testing.assert_allclose(before, want_before, rtol=1e-9)
+        np.testing.assert_allclose(after, want_after, rtol=1e-6, atol=1e-10)
+        assert rel(got, X0 @ C) < 1e-8
+        if kind == "mixed columns":
+            # up to the ridge's bias, ~1e-13 cond(W)^2
+            assert after < 1e-7 and rel(got, X) < 1e-6
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_warm_start_that_already_converged(precision):
     """A guess that already meets the tolerance: the float64 solve runs the
