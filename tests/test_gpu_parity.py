@@ -1019,6 +1019,34 @@ def test_operator_groups_bit_identical_on_random_widths(draw):
     assert torch.equal(dd.X_device, ref.X_device)
 
 
+def test_pass_profile_counts_and_leaves_results_alone():
+    """bdda_profile_enable / bdda_profile_read: with profiling on, every
+    operator application adds one timed launch to each of its five passes
+    (complex128 and complex64 counted apart), the products stay bit for bit
+    what they are unprofiled, and a read clears the counts."""
+    say("pass profile")
+    prob = _problem(L=9)
+    dd = _gpu_dd(prob, "bicgstab", 1e-2, 1)
+    plan = dd._plan
+    g = torch.Generator(device="cuda").manual_seed(3)
+    S = torch.randn(dd._B_dev.shape, dtype=torch.complex128, device="cuda", generator=g)
+    plain = plan.block_mvp(S)
+    plan.profile_read()
+    plan.profile_enable(True)
+    try:
+        outs = [plan.block_mvp(S) for _ in range(3)]
+        prof = plan.profile_read()
+        again = plan.profile_read()
+    finally:
+        plan.profile_enable(False)
+    assert all(torch.equal(o, plain) for o in outs)
+    assert set(prof) == {("fp64", p) for p in ("z_fwd", "y_fwd", "x_conv", "y_inv", "z_inv")}
+    assert all(cnt == 3 and ms > 0 for ms, cnt in prof.values())
+    assert again == {}
+    plan.block_mvp(S)
+    assert plan.profile_read() == {}
+
+
 def test_operator_groups_out_of_range_refused():
     from bdda_amd import _lib
     lib = _lib.load()
