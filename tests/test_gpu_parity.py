@@ -308,6 +308,38 @@ def test_random_sequences_on_one_plan_match_oracle(draw):
     say(f"plan sequence n={n}: " + "; ".join(ops))
 
 
+@pytest.mark.parametrize("n", [(1, 1, 1), (1, 1, 9), (1, 7, 1), (5, 1, 3), (2, 2, 2), (1, 30, 2)])
+def test_one_cell_axes_match_oracle(n):
+    """Cuboids with one-cell axes (a single dipole, a line, a plate): the
+    doubled grid's length-2 axes and the zero planes of the Green tensor
+    (exactly zero for one cell) on the occupied-block and cuboid products,
+    both precisions, against the oracle."""
+    say(f"one-cell axes n={n}")
+    from bl_krylov import bl_krylov as pk
+    from mvp_fft import mvp_fft as pm
+    lf, k = 0.03, 9.0
+    rng = np.random.default_rng(31)
+    ncell = int(np.prod(n))
+    L = 5
+    P = rng.standard_normal((3 * ncell, L)) + 1j * rng.standard_normal((3 * ncell, L))
+    Ao = omvp.fft_init(n, 3, lf, k)
+    ref = omvp.fft_mvp(n, 3, Ao, P)
+    At = pm.fft_init(n, 3, lf, k)
+    got, gotm = pm.fft_mvp(n, 3, At, P), pm.fft_mvp(n, 3, At, P, precision="mixed")
+    if ncell == 1:
+        assert np.array_equal(ref, np.zeros_like(ref)) and np.array_equal(got, ref) and np.array_equal(gotm, ref)
+    else:
+        assert rel(got, ref) < 1e-12 and rel(gotm, ref) < 2e-6
+    address = np.arange(ncell, dtype=np.int64)
+    diag = rng.standard_normal(3 * ncell) + 1j * rng.standard_normal(3 * ncell) + 3.0
+    jpre = 1.0 / diag
+    S = rng.standard_normal((3 * ncell, L)) + 1j * rng.standard_normal((3 * ncell, L))
+    idx = pk._build_scatter_index(address, 3)
+    a = pk._block_mvp(n, 3, At, idx, diag, S, jpre)
+    b = okr.block_mvp(n, 3, Ao, idx, diag, S, jpre)
+    assert rel(a, b) < (1e-12 if ncell > 1 else 5e-12)
+
+
 @pytest.mark.parametrize("n,L", [((36, 4, 30), 23), ((64, 5, 4), 17), ((64, 5, 4), 37), ((3, 4, 96), 29)])
 def test_prime_block_width_runs_one_column_tiles(n, L):
     """A prime L above the tile cap leaves a 1-column tile (Q = 3), whose
