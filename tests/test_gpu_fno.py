@@ -45,6 +45,51 @@ def test_gemm_bf16_exact_layout(shape):
     assert np.max(np.abs(got - ref)) < 1e-5 * scale
 
 
+@pytest.mark.parametrize("draw", range(6))
+def test_gemm_bf16_random_layouts(draw):
+    """Seeded random products: batch 1-4, M, N, K of 1-150, each operand
+    row- or column-major (element strides in either order, padded leading
+    dimensions and a nonzero offset), random alpha and beta; against the
+    float64 product of the bfloat16-rounded operands."""
+    from bdda_amd.fno import Gemm
+    rng = np.random.default_rng(100 + draw)
+    b = int(rng.integers(1, 5))
+    M, N, K = (int(v) for v in rng.integers(1, 151, size=3))
+    alpha, beta = float(rng.uniform(-2, 2)), float(rng.choice([0.0, float(rng.uniform(-2, 2))]))
+    say(f"gemm random draw={draw} b={b} M={M} N={N} K={K}")
+
+    def operand(rows, cols):
+        # logical (b, rows, cols) matrix in a padded buffer, row- or column-major
+        pad, off = int(rng.integers(0, 5)), int(rng.integers(0, 9))
+        if rng.integers(0, 2):
+            ld = cols + pad
+            st = (rows * ld, ld, 1)
+        else:
+            ld = rows + pad
+            st = (cols * ld, 1, ld)
+        buf = rng.standard_normal(off + b * st[0] + 8).astype(np.float32)
+        i, j = np.meshgrid(np.arange(rows), np.arange(cols), indexing="ij")
+        mat = np.stack([buf[off + q * st[0] + i * st[1] + j * st[2]] for q in range(b)])
+        return buf, off, st, mat
+
+    bufA, oa, sa, A = operand(M, K)
+    bufB, ob, sb, Bm = operand(K, N)
+    bufC, oc, sc, C0 = operand(M, N)
+    ta, tb, tc = (torch.from_numpy(x.copy()).cuda() for x in (bufA, bufB, bufC))
+    Gemm()(b, M, N, K, (ta, oa, sa), (tb, ob, sb), (tc, oc, sc), alpha=alpha, beta=beta)
+    out = tc.cpu().numpy()
+    i, j = np.meshgrid(np.arange(M), np.arange(N), indexing="ij")
+    got = np.stack([out[oc + q * sc[0] + i * sc[1] + j * sc[2]] for q in range(b)])
+    ref = alpha * np.einsum('bik,bkj->bij', bf16(A), bf16(Bm)) + beta * C0.astype(np.float64)
+    scale = np.sqrt(K) * np.abs(bf16(A)).max() * np.abs(bf16(Bm)).max() + abs(beta) * np.abs(C0).max()
+    assert np.max(np.abs(got - ref)) < 1e-5 * scale
+    # entries of C's buffer outside the logical matrix are untouched
+    mask = np.ones(bufC.size, dtype=bool)
+    for q in range(b):
+        mask[oc + q * sc[0] + i * sc[1] + j * sc[2]] = False
+    assert np.array_equal(out[mask], bufC[mask])
+
+
 def _tiny(G=8, C=4, K=4, nl=2, hid=8, seed=1):
     from bdda_amd.fno import FNO
     m = FNO(G=G, width=C, modes=K, layers=nl, hidden=hid, seed=seed)
