@@ -155,3 +155,39 @@ def test_sweep_on_the_reference_conditions(tmp_path):
     again = sweep.open_store(path)
     s2 = sweep.run_sweep(again, log=lambda m: None)
     assert s2["computed"] == 0 and s2["skipped"] == 16
+
+
+def test_retries_and_nan_fill():
+    """run_dda's retries (run_dda.py:38-70): a condition whose every try
+    fails returns the last try's orientations, NaN observables (real for C,
+    complex for S) and the try count; one whose first try fails and second
+    converges returns the second try's orientations and results, equal to a
+    direct solve at those orientations."""
+    from bdda_amd import sweep
+    from bl_dda.scatterer import DiscreteDipoles, IncidentField, Target
+    name, n, lf, grid, is_in, r_ve = sweep.build_gre_geometry(0.06, 1.2, 1.1, 0.0, sweep.RNG_SEED)
+    tg = Target(name, n, lf, grid, is_in, np.array([1.5 + 0.01j] * 3))
+    wl, mm, L = 0.4, 1.0, 6
+    tries = sweep.orientation_tries(L, sweep.MAX_TRY, sweep.RNG_SEED + 1)
+    say("sweep retries: every try fails")
+    euler, c_abs, c_ext, s_th, s_ph, s_bk, ok, t = sweep.run_dda(
+        tg, wl, mm, L, log=lambda m: None, options={"tol": 1e-12, "itermax": 1, "verbose": False})
+    assert not ok and t == sweep.MAX_TRY and np.array_equal(euler, tries[-1])
+    assert c_abs.dtype == np.float64 and np.isnan(c_abs).all() and np.isnan(c_ext).all()
+    for s in (s_th, s_ph, s_bk):
+        assert s.dtype == np.complex128 and np.isnan(s.real).all()
+    say("sweep retries: the second try converges")
+    first = DiscreteDipoles(tg, IncidentField(wl, mm, tries[0]))
+    first.verbose, first.tol, first.itermax = False, 1e-12, 1
+    first.set_interaction_matrix()
+    first.solve_matrix_equation()
+    assert not first.converge
+    euler, c_abs, c_ext, s_th, s_ph, s_bk, ok, t = sweep.run_dda(
+        tg, wl, mm, L, log=lambda m: None, options={"verbose": False}, first=first)
+    assert ok and t == 2 and np.array_equal(euler, tries[1])
+    dd = DiscreteDipoles(tg, IncidentField(wl, mm, tries[1]))
+    dd.verbose = False
+    dd.set_interaction_matrix()
+    dd.solve_matrix_equation()
+    assert dd.converge
+    assert np.array_equal(c_ext, dd.compute_C_ext()) and np.array_equal(s_bk, dd.compute_OCBS_observable_S_bk())
