@@ -339,10 +339,11 @@ def batch_leg(torch, nstreams=8):
             "max_true_residual": max(dd.true_residual for dd in dds),
             "iterations": {"min": min(its), "max": max(its), "mean": round(float(np.mean(its)), 2)},
         }
-        # the same batch with every plan's column tile autotuned (tuning is
-        # plan set-up, outside the timed region)
+        # the same batch with every plan's column tile and x pass kernel
+        # autotuned (tuning is plan set-up, outside the timed region)
         t0 = time.perf_counter()
-        tiles = autotune(dds)
+        picks = autotune(dds)
+        tiles, xks = [t for t, _ in picks], [k for _, k in picks]
         torch.cuda.synchronize()
         t_tune = time.perf_counter() - t0
         X_bat = [dd.X_device.clone() for dd in dds]
@@ -355,11 +356,13 @@ def batch_leg(torch, nstreams=8):
         out[prec]["autotuned"] = {
             "value": round(n_or / t_tuned, 2), "batched_s": round(t_tuned, 4), "tuning_s": round(t_tune, 3),
             "tiles": {str(t): tiles.count(t) for t in sorted(set(tiles))},
+            "x_kernels": {{1: "split", 2: "one-pass"}[k]: xks.count(k) for k in sorted(set(xks))},
             "max_rel_diff_vs_untuned": max(float((dd.X_device - x).abs().max() / x.abs().max())
                                            for dd, x in zip(dds, X_bat)),
             "all_converged": all(dd.converge for dd in dds)}
         for dd in dds:
             dd._plan.set_column_tile(0)
+            dd._plan.set_x_kernel(0)
     out["value"] = out[wl["precision"]]["value"]
     return out
 
@@ -515,8 +518,9 @@ def main():
         b = result["batch"]
         log(f"batch: mixed {b['mixed']['value']} / fp64 {b['fp64']['value']} orientations/s "
             f"(x{b['mixed']['speedup_vs_sequential']} / x{b['fp64']['speedup_vs_sequential']} vs one at a time)")
-        log(f"batch, autotuned tiles: mixed {b['mixed']['autotuned']['value']} / fp64 "
-            f"{b['fp64']['autotuned']['value']} orientations/s (tiles {b['fp64']['autotuned']['tiles']})")
+        log(f"batch, autotuned tiles and x kernels: mixed {b['mixed']['autotuned']['value']} / fp64 "
+            f"{b['fp64']['autotuned']['value']} orientations/s (tiles {b['fp64']['autotuned']['tiles']}, "
+            f"x kernels {b['fp64']['autotuned']['x_kernels']})")
 
     if not a.no_cpu_baseline:
         log("cpu baseline")

@@ -33,6 +33,8 @@ _SIGNATURES = {
     "bdda_plan_column_tile": ([_vp, _i], _i),
     "bdda_plan_set_column_tile": ([_vp, _i], _i),
     "bdda_plan_autotune": ([_vp, _i, _i, _pi, _vp], _i),
+    "bdda_plan_set_x_kernel": ([_vp, _i], _i),
+    "bdda_plan_x_kernel": ([_vp], _i),
     "bdda_plan_create": ([_i, _i, _i, ctypes.POINTER(_vp)], _i),
     "bdda_plan_destroy": ([_vp], _i),
     "bdda_plan_fft_dims": ([_vp, _pi], _i),

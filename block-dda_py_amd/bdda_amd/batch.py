@@ -20,10 +20,11 @@ from ._lib import check
 
 
 def autotune(dds):
-    """Fix each instance's column tile to the fastest for its own lattice,
-    L and precision (``bdda_plan_autotune``: the operator timed at each
-    candidate tile); done once per plan, L and precision.  Returns the tiles."""
-    tiles = []
+    """Fix each instance's column tile and x pass kernel to the fastest pair
+    for its own lattice, L and precision (``bdda_plan_autotune``: the
+    operator timed at each candidate); done once per plan, L and precision.
+    Returns the (tile, x kernel) pairs."""
+    picks = []
     for dd in dds:
         key = (dd._B_dev.shape[1], dd.precision)
         done = getattr(dd._plan, "_tuned", {})
@@ -31,9 +32,10 @@ def autotune(dds):
             done[key] = dd._plan.autotune(*key)
             dd._plan._tuned = done
         else:
-            dd._plan.set_column_tile(done[key])
-        tiles.append(done[key])
-    return tiles
+            dd._plan.set_column_tile(done[key][0])
+            dd._plan.set_x_kernel(done[key][1])
+        picks.append(done[key])
+    return picks
 
 
 def solve_batch(dds, nstreams=8):

@@ -85,13 +85,25 @@ class Plan:
         """Fix the column tile (0: automatic)."""
         check(self.lib.bdda_plan_set_column_tile(self.h, int(lb)), "set_column_tile")
 
+    def set_x_kernel(self, kernel):
+        """Fix the x pass kernel: 0 the process-wide rule, 1 split-spectrum
+        where eligible, 2 one-pass, 3 split-spectrum where balanced."""
+        check(self.lib.bdda_plan_set_x_kernel(self.h, int(kernel)), "set_x_kernel")
+
+    def x_kernel(self):
+        k = self.lib.bdda_plan_x_kernel(self.h)
+        if k < 0:
+            raise ValueError("x_kernel: bad plan")
+        return k
+
     def autotune(self, L, precision="fp64"):
-        """Time the operator at the candidate column tiles for L columns and
-        keep the fastest; returns the tile."""
+        """Time the operator at the candidate column tiles, each with the
+        split-spectrum and the one-pass x pass, for L columns and keep the
+        fastest pair; returns (tile, x kernel)."""
         lb = ctypes.c_int(0)
         check(self.lib.bdda_plan_autotune(self.h, int(L), _lib.PRECISION[precision], ctypes.byref(lb), _stream()),
               "autotune")
-        return lb.value
+        return lb.value, self.x_kernel()
 
     def green_tensor(self, lf, k):
         out = np.empty(self.m + (3, 3), dtype=np.complex128)

@@ -1435,7 +1435,8 @@ void launch_reg2_fixed(LinePass lp, hipStream_t s) {
       constexpr int S = Q | 1;
       const size_t lds_x = ((size_t)M * S + M) * sizeof(V);
       const dim3 grid_x(((lp.npts + 7) / 8) * 8 * nlo, 1);
-      const int split = g_xconv_split.load();
+      // the plan's choice, else the process-wide one (0 where balanced, 1 everywhere, 2 nowhere)
+      const int split = lp.xsplit > 0 ? (lp.xsplit == 3 ? 0 : lp.xsplit) : g_xconv_split.load();
       if (split == 1 || (split == 0 && xsplit_balanced<R1, R2>())) {
         launch_xconv_split<V, R1, R2, LB>(lp, grid_x, s);
         return;

@@ -84,6 +84,8 @@ struct LinePass {
   int nyz_y, nz;        // cuboid Ny, Nz (z-line points are (ix, iy))
   int L, Lb;            // total columns, column tile
   int col0;             // first column of this launch's tiles (scatter / gather blocks)
+  int xsplit;           // x pass kernel: 0 the process-wide rule (set_fft_path), 1 split-spectrum
+                        // where eligible, 2 one-pass, 3 split only where R2 = R1 / 2
   int nlo;              // number of column tiles (grid.y)
   const void* S;        // occupied block (3N, L) (scatter source / diag term)
   void* AS;             // occupied output (3N, L)

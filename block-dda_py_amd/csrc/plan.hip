@@ -134,6 +134,8 @@ struct bdda_plan {
   // column tile set by bdda_plan_set_column_tile / bdda_plan_autotune (0:
   // the rule of choose_lb)
   int lb_force = 0;
+  // x pass kernel of this plan's operator (LinePass::xsplit): 0 the process-wide rule
+  int x_kernel = 0;
   // optional per-pass kernel timing (HIP events recorded around each pass)
   bool profiling = false;
   std::vector<hipEvent_t> ev_pool;
@@ -375,6 +377,7 @@ static int apply_operator(bdda_plan* p, const BV* S, BV* AS, int L, bool occupie
     LinePass lp = base;
     axis(lp, 0);
     lp.mode = PASS_CONV_X;
+    lp.xsplit = p->x_kernel;
     lp.nin = ox;
     lp.nout = ox;
     lp.npts = my * mz;
@@ -1692,9 +1695,27 @@ int bdda_plan_set_column_tile(bdda_plan* p, int lb) {
   return 0;
 }
 
+int bdda_plan_set_x_kernel(bdda_plan* p, int kernel) {
+  if (int rc = check_plan(p)) return rc;
+  g_err.clear();
+  if (kernel < 0 || kernel > 3) {
+    set_error("bdda_plan_set_x_kernel: kernel must be 0 (the process-wide rule), 1 (split-spectrum where "
+              "eligible), 2 (one-pass) or 3 (split-spectrum where the radix pair is balanced)");
+    return BDDA_E_ARG;
+  }
+  p->x_kernel = kernel;
+  return 0;
+}
+
+int bdda_plan_x_kernel(const bdda_plan* p) {
+  if (int rc = check_plan(p)) return -rc;
+  return p->x_kernel;
+}
+
 // Times the occupied operator (one warm-up, then three applications between
 // HIP events on the caller's stream) for the column tiles 4, 8 and the
-// automatic choice, within the plan's cap, and keeps the fastest.
+// automatic choice, within the plan's cap, each with the split-spectrum and
+// the one-pass x pass, and keeps the fastest pair.
 int bdda_plan_autotune(bdda_plan* p, int L, int precision, int* lb_out, void* stream) {
   if (int rc = check_plan(p)) return rc;
   g_err.clear();
@@ -1724,10 +1745,12 @@ int bdda_plan_autotune(bdda_plan* p, int L, int precision, int* lb_out, void* st
     set_error("bdda_plan_autotune: hipEventCreate failed");
     return BDDA_E_HIP;
   }
-  int best = cand[0], rc = 0;
+  int best = cand[0], best_xk = 0, rc = 0;
   float best_ms = INFINITY;
-  for (int lb : cand) {
+  for (int c = 0; c < (int)cand.size() * 2 && !rc; ++c) {
+    const int lb = cand[c / 2], xk = 1 + c % 2;
     p->lb_force = lb;
+    p->x_kernel = xk;
     auto apply = [&]() {
       return precision == BDDA_PREC_MIXED
                  ? apply_operator<float2, double2>(p, S.as<double2>(), AS.as<double2>(), L, true, s)
@@ -1747,11 +1770,13 @@ int bdda_plan_autotune(bdda_plan* p, int L, int precision, int* lb_out, void* st
     if (ms < best_ms) {
       best_ms = ms;
       best = lb;
+      best_xk = xk;
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   p->lb_force = rc ? 0 : best;
+  p->x_kernel = rc ? 0 : best_xk;
   if (!rc && lb_out) *lb_out = best;
   return rc;
 }

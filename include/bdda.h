@@ -94,14 +94,27 @@ int bdda_plan_column_tile(const bdda_plan* plan, int L);
  * reference's numpy FFTs have no tiles). */
 int bdda_plan_set_column_tile(bdda_plan* plan, int lb);
 
-/* Autotune the column tile for L columns in `precision`: times the occupied
- * operator (one warm-up and three applications per candidate, on `stream`)
- * at the tiles 4, 8 and the automatic choice, within the plan's limit, and
- * fixes the fastest (*lb_out, nullable).  Needs bdda_green_init and
- * bdda_set_target; allocates and frees one pair of (3N, L) blocks.  The
- * result is the same operator to rounding at any tile.  For batched
- * instances, whose lattices give every instance its own FFT lengths; no
- * reference counterpart. */
+/* The x pass kernel of the plan's operator (the pass that carries the
+ * Green multiply, on the compiled column tiles): 0 follows the process-wide
+ * rule of bdda_set_fft_path (the default), 1 the split-spectrum kernel on
+ * every eligible length, 2 the one-pass kernel, 3 the split-spectrum kernel
+ * where the radix pair is balanced (R2 = R1 / 2) and the one-pass one
+ * elsewhere.  The products agree to rounding.  Returns BDDA_E_ARG for
+ * other values.  No reference counterpart. */
+int bdda_plan_set_x_kernel(bdda_plan* plan, int kernel);
+/* The plan's x pass kernel setting (0-3 as above), or -BDDA_E_ARG. */
+int bdda_plan_x_kernel(const bdda_plan* plan);
+
+/* Autotune the column tile and the x pass kernel for L columns in
+ * `precision`: times the occupied operator (one warm-up and three
+ * applications per candidate, on `stream`) at the tiles 4, 8 and the
+ * automatic choice, within the plan's limit, each with the split-spectrum
+ * (1) and the one-pass (2) x pass, and fixes the fastest pair (the tile in
+ * *lb_out, nullable; the kernel through bdda_plan_x_kernel).  Needs
+ * bdda_green_init and bdda_set_target; allocates and frees one pair of
+ * (3N, L) blocks.  The result is the same operator to rounding at any
+ * setting.  For batched instances, whose lattices give every instance its
+ * own FFT lengths; no reference counterpart. */
 int bdda_plan_autotune(bdda_plan* plan, int L, int precision, int* lb_out, void* stream);
 
 /* Green tensor A'(r) on the doubled grid and its 3-D FFT, kept on device in

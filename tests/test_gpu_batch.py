@@ -71,9 +71,10 @@ def test_batch_rejects_shared_plans(small_batch):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_autotuned_tiles_keep_the_operator(small_batch, precision):
-    """bdda_plan_autotune fixes one of the candidate column tiles; the
-    operator at that tile equals the automatic tile's to rounding, and the
-    batch still converges to the tolerance with the tuned plans."""
+    """bdda_plan_autotune fixes one of the candidate column tiles and x pass
+    kernels; the operator at that setting equals the automatic one's to
+    rounding, and the batch still converges to the tolerance with the tuned
+    plans."""
     from bdda_amd import workloads
     from bdda_amd.batch import autotune, solve_batch
     say(f"autotuned column tiles, {precision}")
@@ -84,14 +85,16 @@ def test_autotuned_tiles_keep_the_operator(small_batch, precision):
         auto = plan.column_tile(L)
         S = torch.randn(dd._B_dev.shape, dtype=torch.complex128, device="cuda", generator=g)
         ref = plan.block_mvp(S)
-        lb = plan.autotune(L, precision)
-        assert lb in {auto, 4, 8} and plan.column_tile(L) == lb
+        lb, xk = plan.autotune(L, precision)
+        assert lb in {auto, 4, 8} and plan.column_tile(L) == lb and xk in {1, 2} and plan.x_kernel() == xk
         got = plan.block_mvp(S)
         assert float((got - ref).abs().max() / ref.abs().max()) < 1e-13
         plan.set_column_tile(0)
-        assert plan.column_tile(L) == auto
-    tiles = autotune(dds)
-    assert all(dd._plan.column_tile(dd._B_dev.shape[1]) == t for dd, t in zip(dds, tiles))
+        plan.set_x_kernel(0)
+        assert plan.column_tile(L) == auto and plan.x_kernel() == 0
+    picks = autotune(dds)
+    assert all(dd._plan.column_tile(dd._B_dev.shape[1]) == lb and dd._plan.x_kernel() == xk
+               for dd, (lb, xk) in zip(dds, picks))
     solve_batch(dds, nstreams=3)
     assert all(dd.converge and dd.true_residual < 1e-5 for dd in dds)
 
