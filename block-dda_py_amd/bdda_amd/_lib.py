@@ -58,6 +58,8 @@ _SIGNATURES = {
     "bdda_inside_mask": ([_vp, _i, _i, _i, _d, _vp], _i),
     "bdda_gemm_bf16": ([_i, _i, _i, _i, ctypes.c_float, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64,
                         ctypes.c_float, _vp, _i64, _i64, _i64, _vp], _i),
+    "bdda_gemm_bf16_ex": ([_i, _i, _i, _i, ctypes.c_float, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64,
+                           ctypes.c_float, _vp, _i64, _i64, _i64, _vp, _i, _vp], _i),
     "bdda_fno_inputs": ([_vp, _vp, _vp, _vp, _i, _i64, _vp], _i),
     "bdda_cube_exchange": ([_vp, _vp, _vp, _vp, _i64, _i, _i, _i, _vp], _i),
     "bdda_masked_l2": ([_vp, _vp, _vp, _vp, _i, _i64, _vp, _vp, _i, _vp], _i),

@@ -49,16 +49,22 @@ class Gemm:
     """C = alpha A B + beta C over `batch` products with element strides.
 
     Operands are (tensor, offset, (batch stride, row stride, col stride)).
+    `bias` (M floats) is added to every row of every product, then act=1
+    applies GELU, in the product's own epilogue.
     """
 
     def __init__(self):
         self.lib = _lib.load()
 
-    def __call__(self, batch, M, N, K, A, B, C, alpha=1.0, beta=0.0):
+    def __call__(self, batch, M, N, K, A, B, C, alpha=1.0, beta=0.0, bias=None, act=0):
         (ta, oa, sa), (tb, ob, sb), (tc, oc, sc) = A, B, C
-        check(self.lib.bdda_gemm_bf16(int(batch), int(M), int(N), int(K), ctypes.c_float(alpha), _p(ta, oa),
-                                      sa[0], sa[1], sa[2], _p(tb, ob), sb[0], sb[1], sb[2], ctypes.c_float(beta),
-                                      _p(tc, oc), sc[0], sc[1], sc[2], _stream()), "gemm_bf16")
+        args = (int(batch), int(M), int(N), int(K), ctypes.c_float(alpha), _p(ta, oa), sa[0], sa[1], sa[2],
+                _p(tb, ob), sb[0], sb[1], sb[2], ctypes.c_float(beta), _p(tc, oc), sc[0], sc[1], sc[2])
+        if bias is None and not act:
+            check(self.lib.bdda_gemm_bf16(*args, _stream()), "gemm_bf16")
+        else:
+            bp = ctypes.c_void_p(None) if bias is None else _p(bias)
+            check(self.lib.bdda_gemm_bf16_ex(*args, bp, int(act), _stream()), "gemm_bf16")
 
 
 class FNO:
@@ -79,6 +85,12 @@ class FNO:
         dev = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=F32, device="cuda")
         self.Fr, self.Fim = dev(F.real), dev(F.imag)
         self.Fir, self.Fii = dev(Fi.real), dev(Fi.imag)
+        # the z transforms of the forward pass as one product each way, so the
+        # activations are read (and the output written) once: F's real and
+        # imaginary rows stacked, (2K, G); Fi's real and negated imaginary
+        # columns side by side, (G, 2K)
+        self.Fz = dev(np.concatenate([F.real, F.imag], axis=0))
+        self.Fzi = dev(np.concatenate([Fi.real, -Fi.imag], axis=1))
         self.params = {}
         if params is None:
             rng = np.random.default_rng(seed)
@@ -108,10 +120,12 @@ class FNO:
     def _empty(self, *shape):
         return torch.empty(shape, dtype=F32, device="cuda")
 
-    def _pointwise(self, v, W, out, B, ci, co, beta=0.0):
-        """out[b] (co, S) = W (co, ci) @ v[b] (ci, S)  (+ beta out)."""
+    def _pointwise(self, v, W, out, B, ci, co, beta=0.0, bias=None, act=0):
+        """out[b] (co, S) = W (co, ci) @ v[b] (ci, S)  (+ beta out) (+ bias),
+        then GELU if act."""
         S = self.S
-        self.gemm(B, co, S, ci, (W, 0, (0, ci, 1)), (v, 0, (ci * S, S, 1)), (out, 0, (co * S, S, 1)), beta=beta)
+        self.gemm(B, co, S, ci, (W, 0, (0, ci, 1)), (v, 0, (ci * S, S, 1)), (out, 0, (co * S, S, 1)), beta=beta,
+                  bias=bias, act=act)
 
     def _pointwise_bwd(self, v, W, dout, dv, dW, B, ci, co, dv_beta):
         S = self.S
@@ -144,25 +158,27 @@ class FNO:
 
     # ----------------------------------------------------- spectral layer
     def _cmm(self, batch, M, N, K, Ar, Ai, sa, Br, Bi, sb, Cr, Ci, sc, conjA=False):
-        """Complex (Ar + i Ai)(Br + i Bi) (conjA: conj(A)) into (Cr, Ci)."""
+        """Complex (Ar + i Ai)(Br + i Bi) (conjA: conj(A)) into (Cr, Ci).
+        Each part is a tensor or a (tensor, element offset) pair."""
         g = self.gemm
         s = -1.0 if conjA else 1.0
-        g(batch, M, N, K, (Ar, 0, sa), (Br, 0, sb), (Cr, 0, sc))
-        g(batch, M, N, K, (Ai, 0, sa), (Bi, 0, sb), (Cr, 0, sc), alpha=-s, beta=1.0)
-        g(batch, M, N, K, (Ar, 0, sa), (Bi, 0, sb), (Ci, 0, sc))
-        g(batch, M, N, K, (Ai, 0, sa), (Br, 0, sb), (Ci, 0, sc), alpha=s, beta=1.0)
+        Ar, Ai, Br, Bi, Cr, Ci = (x if isinstance(x, tuple) else (x, 0) for x in (Ar, Ai, Br, Bi, Cr, Ci))
+        g(batch, M, N, K, (*Ar, sa), (*Br, sb), (*Cr, sc))
+        g(batch, M, N, K, (*Ai, sa), (*Bi, sb), (*Cr, sc), alpha=-s, beta=1.0)
+        g(batch, M, N, K, (*Ar, sa), (*Bi, sb), (*Ci, sc))
+        g(batch, M, N, K, (*Ai, sa), (*Br, sb), (*Ci, sc), alpha=s, beta=1.0)
 
     def _spectral_fwd(self, v, Rr, Ri, B, out):
         G, K, C, Km = self.G, self.K, self.C, self.Km
         g = self.gemm
-        # z: real (B C G G, G) x F^T -> (B C G G, K)
-        Xzr, Xzi = self._empty(B * C * G * G * K), self._empty(B * C * G * G * K)
-        g(1, B * C * G * G, K, G, (v, 0, (0, G, 1)), (self.Fr, 0, (0, 1, G)), (Xzr, 0, (0, K, 1)))
-        g(1, B * C * G * G, K, G, (v, 0, (0, G, 1)), (self.Fim, 0, (0, 1, G)), (Xzi, 0, (0, K, 1)))
+        # z: real (B C G G, G) x [F_re; F_im]^T -> (B C G G, 2K), each row its K
+        # real parts, then its K imaginary parts
+        Xz = self._empty(B * C * G * G * 2 * K)
+        g(1, B * C * G * G, 2 * K, G, (v, 0, (0, G, 1)), (self.Fz, 0, (0, 1, G)), (Xz, 0, (0, 2 * K, 1)))
         # y
         Xyr, Xyi = self._empty(B * C * G * K * K), self._empty(B * C * G * K * K)
-        self._cmm(B * C * G, K, K, G, self.Fr, self.Fim, (0, G, 1), Xzr, Xzi, (G * K, K, 1), Xyr, Xyi,
-                  (K * K, K, 1))
+        self._cmm(B * C * G, K, K, G, self.Fr, self.Fim, (0, G, 1), (Xz, 0), (Xz, K), (G * 2 * K, 2 * K, 1), Xyr,
+                  Xyi, (K * K, K, 1))
         # x
         Xhr, Xhi = self._empty(B * C * Km), self._empty(B * C * Km)
         self._cmm(B * C, K, K * K, G, self.Fr, self.Fim, (0, G, 1), Xyr, Xyi, (G * K * K, K * K, 1), Xhr, Xhi,
@@ -174,14 +190,12 @@ class FNO:
         Yxr, Yxi = self._empty(B * C * G * K * K), self._empty(B * C * G * K * K)
         self._cmm(B * C, G, K * K, K, self.Fir, self.Fii, (0, K, 1), Yhr, Yhi, (Km, K * K, 1), Yxr, Yxi,
                   (G * K * K, K * K, 1))
-        # inverse y
-        Yyr, Yyi = self._empty(B * C * G * G * K), self._empty(B * C * G * G * K)
-        self._cmm(B * C * G, G, K, K, self.Fir, self.Fii, (0, K, 1), Yxr, Yxi, (K * K, K, 1), Yyr, Yyi,
-                  (G * K, K, 1))
-        # inverse z, real part
-        g(1, B * C * G * G, G, K, (Yyr, 0, (0, K, 1)), (self.Fir, 0, (0, 1, K)), (out, 0, (0, G, 1)))
-        g(1, B * C * G * G, G, K, (Yyi, 0, (0, K, 1)), (self.Fii, 0, (0, 1, K)), (out, 0, (0, G, 1)), alpha=-1.0,
-          beta=1.0)
+        # inverse y, into rows of K real parts then K imaginary parts
+        Yy = self._empty(B * C * G * G * 2 * K)
+        self._cmm(B * C * G, G, K, K, self.Fir, self.Fii, (0, K, 1), Yxr, Yxi, (K * K, K, 1), (Yy, 0), (Yy, K),
+                  (G * 2 * K, 2 * K, 1))
+        # inverse z, real part: (B C G G, 2K) x [Fi_re, -Fi_im]^T
+        g(1, B * C * G * G, G, 2 * K, (Yy, 0, (0, 2 * K, 1)), (self.Fzi, 0, (0, 1, 2 * K)), (out, 0, (0, G, 1)))
         return Xhr, Xhi
 
     def _spectral_bwd(self, dout, Xhr, Xhi, Rr, Ri, B, dv, dRr, dRi):
@@ -228,30 +242,32 @@ class FNO:
         B = x.shape[0]
         C, S = self.C, self.S
         cache = {"x": x}
+        # biases (and, where no pre-activation is kept for the backward
+        # pass, the GELUs) run in the pointwise products' epilogues
         v = self._empty(B, C, S)
-        self._pointwise(x, P["lift.W"], v, B, self.cin, C)
-        self._bias_add(v, P["lift.b"], B, C)
+        self._pointwise(x, P["lift.W"], v, B, self.cin, C, bias=P["lift.b"])
         for l in range(self.nl):
+            last = l == self.nl - 1
             z = self._empty(B, C, S)
             Xh = self._spectral_fwd(v, P[f"l{l}.Rr"], P[f"l{l}.Ri"], B, z)
-            self._pointwise(v, P[f"l{l}.W"], z, B, C, C, beta=1.0)
-            self._bias_add(z, P[f"l{l}.b"], B, C)
+            self._pointwise(v, P[f"l{l}.W"], z, B, C, C, beta=1.0, bias=P[f"l{l}.b"], act=int(not keep and not last))
             if keep:
                 cache[f"v{l}"], cache[f"z{l}"], cache[f"Xh{l}"] = v, z, Xh
-            if l < self.nl - 1:
+            if keep and not last:
                 vn = self._empty(B, C, S)
                 self._gelu(z, vn)
                 v = vn
             else:
                 v = z
         h = self._empty(B, self.hid, S)
-        self._pointwise(v, P["p1.W"], h, B, C, self.hid)
-        self._bias_add(h, P["p1.b"], B, self.hid)
-        hg = self._empty(B, self.hid, S)
-        self._gelu(h, hg)
+        self._pointwise(v, P["p1.W"], h, B, C, self.hid, bias=P["p1.b"], act=int(not keep))
+        if keep:
+            hg = self._empty(B, self.hid, S)
+            self._gelu(h, hg)
+        else:
+            hg = h
         y = self._empty(B, self.cout, S)
-        self._pointwise(hg, P["p2.W"], y, B, self.hid, self.cout)
-        self._bias_add(y, P["p2.b"], B, self.cout)
+        self._pointwise(hg, P["p2.W"], y, B, self.hid, self.cout, bias=P["p2.b"])
         if keep:
             cache.update(vL=v, h=h, hg=hg)
             self.cache = cache

@@ -161,6 +161,8 @@ struct GemmArgs {
   long long sBb, sBk, sBj;
   float* C;
   long long sCb, sCi, sCj;
+  const float* bias;  // per row i of C, added after the product (nullable)
+  int act;            // 1: GELU of the result (after the bias), 0: none
 };
 void launch_gemm_bf16(const GemmArgs& g, hipStream_t s);
 void launch_gelu(const float* x, float* y, long long n, hipStream_t s);

@@ -85,7 +85,10 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs g) {
         if (r < g.M && c < g.N) {
           float* o = C + r * g.sCi + (long long)c * g.sCj;
           const float prev = g.beta != 0.f ? *o : 0.f;
-          *o = g.alpha * acc[ti][tj][j] + g.beta * prev;
+          float val = g.alpha * acc[ti][tj][j] + g.beta * prev;
+          if (g.bias) val += g.bias[r];
+          if (g.act) val = 0.5f * val * (1.f + erff(val * 0.70710678118654752f));
+          *o = val;
         }
       }
 }

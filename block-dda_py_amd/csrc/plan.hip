@@ -2246,13 +2246,20 @@ int bdda_inside_mask(const double* dist, int nx, int ny, int nz, double thresh, 
 int bdda_gemm_bf16(int batch, int M, int N, int K, float alpha, const float* A, int64_t sAb, int64_t sAi,
                    int64_t sAk, const float* B, int64_t sBb, int64_t sBk, int64_t sBj, float beta, float* C,
                    int64_t sCb, int64_t sCi, int64_t sCj, void* stream) {
+  return bdda_gemm_bf16_ex(batch, M, N, K, alpha, A, sAb, sAi, sAk, B, sBb, sBk, sBj, beta, C, sCb, sCi, sCj,
+                           nullptr, 0, stream);
+}
+
+int bdda_gemm_bf16_ex(int batch, int M, int N, int K, float alpha, const float* A, int64_t sAb, int64_t sAi,
+                      int64_t sAk, const float* B, int64_t sBb, int64_t sBk, int64_t sBj, float beta, float* C,
+                      int64_t sCb, int64_t sCi, int64_t sCj, const float* bias, int act, void* stream) {
   g_err.clear();
-  if (batch < 0 || M < 0 || N < 0 || K < 0 || !A || !B || !C) {
+  if (batch < 0 || M < 0 || N < 0 || K < 0 || !A || !B || !C || (act != 0 && act != 1)) {
     set_error("bdda_gemm_bf16: bad arguments");
     return BDDA_E_ARG;
   }
   if (batch == 0 || M == 0 || N == 0) return 0;
-  GemmArgs g{0, M, N, K, alpha, beta, A, sAb, sAi, sAk, B, sBb, sBk, sBj, C, sCb, sCi, sCj};
+  GemmArgs g{0, M, N, K, alpha, beta, A, sAb, sAi, sAk, B, sBb, sBk, sBj, C, sCb, sCi, sCj, bias, act};
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   for (int b0 = 0; b0 < batch; b0 += 65535) {
     g.batch = std::min(65535, batch - b0);

@@ -307,6 +307,13 @@ int bdda_inside_mask(const double* dist_host, int nx, int ny, int nz, double thr
 int bdda_gemm_bf16(int batch, int M, int N, int K, float alpha, const float* A, int64_t sAb, int64_t sAi,
                    int64_t sAk, const float* B, int64_t sBb, int64_t sBk, int64_t sBj, float beta, float* C,
                    int64_t sCb, int64_t sCi, int64_t sCj, void* stream);
+/* The same product with an epilogue: bias (nullable, M floats) is added to
+ * every row i of every product, then act = 1 applies GELU (act = 0: none).
+ * Equal bit for bit to bdda_gemm_bf16 followed by bdda_bias_add and
+ * bdda_gelu, without their passes over C.  BDDA_E_ARG for other act. */
+int bdda_gemm_bf16_ex(int batch, int M, int N, int K, float alpha, const float* A, int64_t sAb, int64_t sAi,
+                      int64_t sAk, const float* B, int64_t sBb, int64_t sBk, int64_t sBj, float beta, float* C,
+                      int64_t sCb, int64_t sCi, int64_t sCj, const float* bias, int act, void* stream);
 /* y = GELU(x) = 0.5 x (1 + erf(x / sqrt 2)); dx = dy * GELU'(x). */
 int bdda_gelu(const float* x, float* y, int64_t n, void* stream);
 int bdda_gelu_bwd(const float* x, const float* dy, float* dx, int64_t n, void* stream);

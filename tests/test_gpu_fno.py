@@ -90,6 +90,46 @@ def test_gemm_bf16_random_layouts(draw):
     assert np.array_equal(out[mask], bufC[mask])
 
 
+@pytest.mark.parametrize("act", [0, 1])
+@pytest.mark.parametrize("beta", [0.0, 1.0])
+def test_gemm_epilogue_equals_separate_passes(act, beta):
+    """The product's bias (+ GELU) epilogue against the product followed by
+    bdda_bias_add (and bdda_gelu): equal bit for bit, on the pointwise
+    channel-mixing shape (batch 3, 37 output channels, 1000 cells, 29 input
+    channels)."""
+    from bdda_amd import _lib
+    from bdda_amd.fno import Gemm, _p, _stream
+    say(f"gemm epilogue act={act} beta={beta}")
+    lib = _lib.load()
+    b, co, ci, S = 3, 37, 29, 1000
+    rng = np.random.default_rng(7 + act)
+    W = torch.tensor(rng.standard_normal((co, ci)), dtype=torch.float32, device="cuda")
+    v = torch.tensor(rng.standard_normal((b, ci, S)), dtype=torch.float32, device="cuda")
+    c0 = torch.tensor(rng.standard_normal((b, co, S)), dtype=torch.float32, device="cuda")
+    bias = torch.tensor(rng.standard_normal(co), dtype=torch.float32, device="cuda")
+    ops = ((W, 0, (0, ci, 1)), (v, 0, (ci * S, S, 1)))
+    sep, fused = c0.clone(), c0.clone()
+    Gemm()(b, co, S, ci, *ops, (sep, 0, (co * S, S, 1)), beta=beta)
+    _lib.check(lib.bdda_bias_add(_p(sep), _p(bias), b, co, S, _stream()), "bias_add")
+    if act:
+        out = torch.empty_like(sep)
+        _lib.check(lib.bdda_gelu(_p(sep), _p(out), sep.numel(), _stream()), "gelu")
+        sep = out
+    Gemm()(b, co, S, ci, *ops, (fused, 0, (co * S, S, 1)), beta=beta, bias=bias, act=act)
+    assert torch.equal(sep, fused)
+
+
+def test_forward_with_and_without_backward_cache_agree():
+    """The inference forward (GELUs in the product epilogues) and the
+    training forward (pre-activations kept, separate GELU passes) give the
+    same outputs bit for bit."""
+    say("fno forward paths")
+    m, _ = _tiny()
+    rng = np.random.default_rng(5)
+    x = torch.tensor(rng.standard_normal((2, 14, 8 ** 3)), dtype=torch.float32, device="cuda")
+    assert torch.equal(m.forward(x), m.forward(x, keep=True))
+
+
 def _tiny(G=8, C=4, K=4, nl=2, hid=8, seed=1):
     from bdda_amd.fno import FNO
     m = FNO(G=G, width=C, modes=K, layers=nl, hidden=hid, seed=seed)
