@@ -99,3 +99,21 @@ def test_batch_entry_validates_arguments_without_a_gpu():
     Ls = (ctypes.c_int * 2)(4, 4)
     rc = lib.bdda_solve_batch(2, fake, 0, 0, bufs, None, bufs, Ls, 1e-5, 10, 20, 2, nul, nd, nd, nd, nul)
     assert rc == 1 and b"own" in lib.bdda_last_error()
+
+
+def test_plan_and_gemm_entries_validate_arguments_without_a_gpu():
+    """The per-plan settings refuse a null plan; the GEMM entry points refuse
+    an unknown epilogue and return at once on an empty batch, all before
+    touching the device."""
+    import ctypes
+    from bdda_amd import _lib
+    lib = _lib.load()
+    assert lib.bdda_plan_set_x_kernel(None, 1) == 1 and b"null plan" in lib.bdda_last_error()
+    assert lib.bdda_plan_x_kernel(None) == -1
+    assert lib.bdda_plan_set_column_tile(None, 4) == 1
+    fake = ctypes.c_void_p(0x1000)
+    args = (1, 4, 4, 4, ctypes.c_float(1.0), fake, 0, 4, 1, fake, 0, 4, 1, ctypes.c_float(0.0), fake, 0, 4, 1)
+    assert lib.bdda_gemm_bf16_ex(*args, None, 2, None) == 1 and b"bad arguments" in lib.bdda_last_error()
+    empty = (0,) + args[1:]
+    assert lib.bdda_gemm_bf16(*empty, None) == 0
+    assert lib.bdda_gemm_bf16_ex(*empty, None, 1, None) == 0
