@@ -66,6 +66,7 @@ _SIGNATURES = {
     "bdda_gelu": ([_vp, _vp, _i64, _vp], _i),
     "bdda_gelu_bwd": ([_vp, _vp, _vp, _i64, _vp], _i),
     "bdda_bias_add": ([_vp, _vp, _i64, _i, _i64, _vp], _i),
+    "bdda_fno_ycombine": ([_vp, _vp, _vp, _i64, _i, _vp], _i),
     "bdda_bias_grad": ([_vp, _vp, _i64, _i, _i64, _vp], _i),
     "bdda_adam": ([_vp, _vp, _vp, _vp, _i64, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _i,
                    _vp], _i),

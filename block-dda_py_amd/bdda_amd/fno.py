@@ -171,14 +171,17 @@ class FNO:
     def _spectral_fwd(self, v, Rr, Ri, B, out):
         G, K, C, Km = self.G, self.K, self.C, self.Km
         g = self.gemm
-        # z: real (B C G G, G) x [F_re; F_im]^T -> (B C G G, 2K), each row its K
-        # real parts, then its K imaginary parts
-        Xz = self._empty(B * C * G * G * 2 * K)
-        g(1, B * C * G * G, 2 * K, G, (v, 0, (0, G, 1)), (self.Fz, 0, (0, 1, G)), (Xz, 0, (0, 2 * K, 1)))
-        # y
-        Xyr, Xyi = self._empty(B * C * G * K * K), self._empty(B * C * G * K * K)
-        self._cmm(B * C * G, K, K, G, self.Fr, self.Fim, (0, G, 1), (Xz, 0), (Xz, K), (G * 2 * K, 2 * K, 1), Xyr,
-                  Xyi, (K * K, K, 1))
+        # z: real (G; B C G, G) x [F_re; F_im]^T -> Z (G, B C G, 2K), y outermost,
+        # each row its K real parts, then its K imaginary parts
+        Q = B * C * G
+        Xz = self._empty(G * Q * 2 * K)
+        g(G, Q, 2 * K, G, (v, 0, (G, G * G, 1)), (self.Fz, 0, (0, 1, G)), (Xz, 0, (Q * 2 * K, 2 * K, 1)))
+        # y: one wide product [F_re; F_im] (2K, G) x Z (G, Q 2K), then the
+        # complex combination into (Q, K, K) planes
+        R = self._empty(2 * K * Q * 2 * K)
+        g(1, 2 * K, Q * 2 * K, G, (self.Fz, 0, (0, G, 1)), (Xz, 0, (0, Q * 2 * K, 1)), (R, 0, (0, Q * 2 * K, 1)))
+        Xyr, Xyi = self._empty(Q * K * K), self._empty(Q * K * K)
+        check(self.lib.bdda_fno_ycombine(_p(R), _p(Xyr), _p(Xyi), Q, K, _stream()), "fno_ycombine")
         # x
         Xhr, Xhi = self._empty(B * C * Km), self._empty(B * C * Km)
         self._cmm(B * C, K, K * K, G, self.Fr, self.Fim, (0, G, 1), Xyr, Xyi, (G * K * K, K * K, 1), Xhr, Xhi,

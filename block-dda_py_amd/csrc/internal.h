@@ -168,6 +168,7 @@ void launch_gemm_bf16(const GemmArgs& g, hipStream_t s);
 void launch_gelu(const float* x, float* y, long long n, hipStream_t s);
 void launch_gelu_bwd(const float* x, const float* dy, float* dx, long long n, hipStream_t s);
 void launch_bias_add(float* y, const float* b, long long B, int C, long long S, hipStream_t s);
+void launch_fno_ycombine(const float* R, float* xr, float* xi, long long nq, int K, hipStream_t s);
 void launch_bias_grad(const float* dy, float* db, long long B, int C, long long S, hipStream_t s);
 void launch_adam(float* p, const float* g, float* m, float* v, long long n, float lr, float b1, float b2, float eps,
                  float c1, float c2, hipStream_t s);

@@ -267,6 +267,29 @@ void launch_masked_l2(const float* pred, const float* target, const unsigned cha
                      nsamp);
 }
 
+// The y DFT of the spectral layer as one wide product gives, per
+// (sample, channel, x) slab q, R[r][q][c]: rows r < K hold F_re times the
+// slab's z spectrum, rows K + r F_im times it; columns c < K its real parts,
+// K + c its imaginary parts.  The complex product is then
+// X_re[q][r][c] = R[r][q][c] - R[K + r][q][K + c],
+// X_im[q][r][c] = R[r][q][K + c] + R[K + r][q][c].
+__global__ void k_fno_ycombine(const float* __restrict__ R, float* __restrict__ xr, float* __restrict__ xi,
+                               long long nq, int K) {
+  const long long kk = (long long)K * K, n = nq * kk, rs = nq * 2 * K;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long q = i / kk;
+    const int r = (int)(i % kk) / K, c = (int)(i % K);
+    const float* a = R + r * rs + q * 2 * K;
+    const float* b = a + K * rs;
+    xr[i] = a[c] - b[K + c];
+    xi[i] = a[K + c] + b[c];
+  }
+}
+void launch_fno_ycombine(const float* R, float* xr, float* xi, long long nq, int K, hipStream_t s) {
+  hipLaunchKernelGGL(k_fno_ycombine, dim3(4096), dim3(256), 0, s, R, xr, xi, nq, K);
+}
+
 void launch_gelu(const float* x, float* y, long long n, hipStream_t s) {
   hipLaunchKernelGGL(k_gelu, dim3(2048), dim3(256), 0, s, x, y, n);
 }

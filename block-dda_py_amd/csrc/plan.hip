@@ -2327,6 +2327,18 @@ int bdda_bias_add(float* y, const float* b, int64_t B, int C, int64_t S, void* s
   return 0;
 }
 
+int bdda_fno_ycombine(const float* R, float* xr, float* xi, int64_t nq, int K, void* stream) {
+  g_err.clear();
+  if (!R || !xr || !xi || nq < 0 || K < 1) {
+    set_error("bdda_fno_ycombine: bad arguments");
+    return BDDA_E_ARG;
+  }
+  if (nq == 0) return 0;
+  launch_fno_ycombine(R, xr, xi, nq, K, reinterpret_cast<hipStream_t>(stream));
+  BDDA_HIP(hipGetLastError());
+  return 0;
+}
+
 int bdda_bias_grad(const float* dy, float* db, int64_t B, int C, int64_t S, void* stream) {
   launch_bias_grad(dy, db, B, C, S, reinterpret_cast<hipStream_t>(stream));
   BDDA_HIP(hipGetLastError());

@@ -320,6 +320,11 @@ int bdda_gelu_bwd(const float* x, const float* dy, float* dx, int64_t n, void* s
 /* (B, C, S) tensors: y[b][c][s] += bias[c];  db[c] = sum_{b,s} dy[b][c][s]. */
 int bdda_bias_add(float* y, const float* bias, int64_t B, int C, int64_t S, void* stream);
 int bdda_bias_grad(const float* dy, float* db, int64_t B, int C, int64_t S, void* stream);
+/* The network's y DFT as one wide product R = [F_re; F_im] Z (2K rows; Z's
+ * nq slabs of 2K columns, K real parts then K imaginary parts) combined into
+ * complex (nq, K, K) planes: xr[q][r][c] = R[r][q][c] - R[K+r][q][K+c],
+ * xi[q][r][c] = R[r][q][K+c] + R[K+r][q][c], R row-major (2K, nq * 2K). */
+int bdda_fno_ycombine(const float* R, float* xr, float* xi, int64_t nq, int K, void* stream);
 /* Network input channels for nsamp samples on a G^3 cube (out: nsamp x 14
  * x G^3).  occ: per-instance occupancy cubes (bytes); sample_inst: instance
  * of each sample; desc: 20 floats per sample (chi re/im x3, u (3), e0 re/im
